@@ -1,0 +1,180 @@
+/*
+ * sky_hip.h -- C ABI of the MI355X spectral sky path tracer (libsky_hip.so).
+ *
+ * Drop-in boundary for the reference's integrator entry point
+ *   sky_spectral_path_tracer::integrator::render(&SceneData, &RenderConfig)
+ *       -> Result<Film, RenderError>
+ * (crates/sky-spectral-path-tracer/src/integrator.rs:125-128, re-exported at
+ * lib.rs:7).  Plain C types, caller-owned buffers, no torch types.
+ *
+ * Threading: a context is single-threaded (not reentrant); distinct contexts
+ * may be used concurrently from different threads.  The library never keeps
+ * caller pointers after a call returns.
+ */
+#ifndef SKY_HIP_H
+#define SKY_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SKY_HIP_ABI_VERSION 1
+
+/* RenderError (integrator.rs:19-26) as status codes. */
+typedef enum sky_status {
+    SKY_OK = 0,
+    SKY_ERR_INVALID_CONFIG = 1, /* RenderError::InvalidConfig                   */
+    SKY_ERR_NO_DEVICE = 2,      /* RenderError::NoAdapter                       */
+    SKY_ERR_DEVICE_INIT = 3,    /* RenderError::RequestDevice                   */
+    SKY_ERR_COPY = 4,           /* RenderError::BufferMap                       */
+    SKY_ERR_SYNC = 5,           /* RenderError::Poll                            */
+    SKY_ERR_WATCHDOG = 6,       /* RenderError::Watchdog                        */
+    SKY_ERR_OOM = 7             /* device allocation failed (no reference twin) */
+} sky_status;
+
+/* Estimator flags (sky_render_config.flags). */
+#define SKY_FLAG_PHASE_IS 1u       /* Rayleigh/HG lobe sampling instead of isotropic (sky_trace.wgsl:735-738) */
+#define SKY_FLAG_RATIO_TRACKING 2u /* ratio tracking for shadow rays instead of delta (sky_trace.wgsl:441-475) */
+#define SKY_FLAG_CARRIED_TAU 4u    /* carry the free-flight optical depth across majorant shells */
+#define SKY_FLAG_SIMPLE_KERNEL 256u /* one-thread-per-(pixel, lambda) kernel instead of the wave-pool kernel */
+
+/* OutputProjection (config.rs:14-36). */
+#define SKY_PROJECTION_PANORAMA 0u
+#define SKY_PROJECTION_SKY_VIEW_LUT 1u
+
+/* Opaque: device, stream, cached device buffers. */
+typedef struct sky_context sky_context;
+
+/* Planet (crates/sky-core/src/geometry.rs:6-17). */
+typedef struct sky_planet {
+    double ground_radius_km;
+    double atmosphere_radius_km;
+} sky_planet;
+
+/* Analytic replacement of the tabulated SceneData medium
+ * (crates/sky-core/src/atmosphere.rs:121-133, medium.rs:28-55):
+ *   sigma_R = rayleigh_sca_550 (550/l)^4 exp(-h/H_R)
+ *   sigma_O = ozone_peak exp(-(l-l0)^2/(2 s^2)) max(0, 1-|h-h_c|/h_w)
+ *   sigma_As = aerosol_sca_550 (l/550)^-angstrom exp(-h/H_A), sigma_Aa likewise
+ * HG aerosol phase (g), Lambertian ground, Planck sun of the given radius. */
+typedef struct sky_atmosphere {
+    double rayleigh_sca_550_m_inv; /* 1.3558e-5 */
+    double rayleigh_h_km;          /* 8         */
+    double ozone_peak_m_inv;       /* 1.9e-6    */
+    double ozone_center_km;        /* 25        */
+    double ozone_half_width_km;    /* 15        */
+    double ozone_lambda0_nm;       /* 600       */
+    double ozone_sigma_nm;         /* 80        */
+    double aerosol_sca_550_m_inv;  /* 3.996e-6  */
+    double aerosol_abs_550_m_inv;  /* 4.4e-6    */
+    double aerosol_h_km;           /* 1.2       */
+    double aerosol_g;              /* 0.8       */
+    double angstrom;               /* 1.3       */
+    double ground_albedo;          /* 0.3       */
+    double sun_temperature_k;      /* 5778      */
+    double sun_angular_radius_rad; /* 0.2675 deg */
+} sky_atmosphere;
+
+/* Wavelength set (replaces the fixed 41 bands, spectrum.rs:5).  weight_nm
+ * multiplies cmf(lambda) in the in-kernel XYZ sum (NULL: uniform spacing). */
+typedef struct sky_wavelengths {
+    uint32_t count; /* 1..64 */
+    const double *center_nm;
+    const double *weight_nm;
+} sky_wavelengths;
+
+/* RenderConfig (config.rs:2-12) plus the extensions. */
+typedef struct sky_render_config {
+    uint32_t width, height, spp;
+    uint32_t sample_offset; /* first sample index: resume / shard point */
+    uint64_t seed;          /* RNG key: seed_lo ^ pixel, seed_hi ^ sample */
+    double sun_elevation_deg, sun_azimuth_deg, observer_altitude_km;
+    uint32_t direct_light_samples; /* >= 1 */
+    uint32_t projection;           /* SKY_PROJECTION_*                 */
+    uint32_t max_depth;            /* 0 = uncapped (reference)         */
+    uint32_t rr_start_depth;       /* 4 in the reference               */
+    uint32_t watchdog_limit;       /* 1024 in the reference            */
+    uint32_t max_segments;         /* 100000 in the reference          */
+    uint32_t majorant_layers;      /* 64 in the reference (data.rs:48) */
+    uint32_t flags;                /* SKY_FLAG_*                       */
+    double majorant_margin;        /* 1.05 in the reference            */
+} sky_render_config;
+
+typedef struct sky_render_stats {
+    double kernel_ms;      /* device time of the integrator launch(es)  */
+    double total_ms;       /* host wall time of the call                */
+    uint64_t path_samples; /* width * height * spp * count              */
+    uint32_t watchdog;     /* nonzero if a watchdog tripped             */
+    uint32_t workgroups;   /* launch shape actually used                */
+    uint32_t block;
+    uint32_t items;
+} sky_render_stats;
+
+/* Launch tuning of the wave-pool integrator (autotuner output); zero fields
+ * mean "use the default". */
+typedef struct sky_tuning {
+    uint32_t waves_per_cu;    /* persistent waves per compute unit (16)        */
+    uint32_t pixels_per_item; /* pixels per wave work item (128 / count)       */
+    uint32_t event_threshold; /* lanes waiting before events are handled (32)  */
+    uint32_t track_steps;     /* tracking micro-steps between handler passes (8) */
+} sky_tuning;
+
+/* ---- context ---- */
+sky_status sky_context_create(int device, sky_context **out);
+void sky_context_destroy(sky_context *ctx);
+sky_status sky_context_set_tuning(sky_context *ctx, const sky_tuning *tuning);
+sky_status sky_context_get_tuning(const sky_context *ctx, sky_tuning *tuning);
+
+/* ---- defaults (config.rs:38-52, BASELINE analytic atmosphere) ---- */
+void sky_default_planet(sky_planet *planet);
+void sky_default_atmosphere(sky_atmosphere *atm);
+void sky_default_config(sky_render_config *cfg);
+
+/* ---- scene build (data.rs:15-51; integrator.rs:319-418 PackedScene) ----
+ * Per-wavelength table, row = [beta_r, beta_o, beta_as, beta_aa (km^-1),
+ * sun_radiance (W m^-2 sr^-1 nm^-1), wx, wy, wz]; n_out >= 8 * count. */
+sky_status sky_wavelength_table(const sky_atmosphere *atm, const sky_wavelengths *wl, float *out,
+                                uint32_t n_out);
+/* Majorant grid (medium.rs:63-83): [lambda * layers + layer], km^-1. */
+sky_status sky_majorant_grid(const sky_planet *planet, const sky_atmosphere *atm,
+                             const sky_wavelengths *wl, uint32_t layers, double margin, float *out,
+                             uint32_t n_out);
+
+/* validate_config (integrator.rs:130-163) for the generalised inputs:
+ * SKY_OK or SKY_ERR_INVALID_CONFIG with the reason in sky_last_error(). */
+sky_status sky_validate_config(const sky_planet *planet, const sky_atmosphere *atm,
+                               const sky_wavelengths *wl, const sky_render_config *cfg);
+
+/* ---- the integrator: render() (integrator.rs:125-128) ----
+ * Host buffers, any may be NULL:
+ *   film    [(y*W + x) * count + band]  mean radiance, clamped >= 0 (integrator.rs:307-310)
+ *   film_m2 same layout, mean of squared path radiance (for standard errors)
+ *   xyz     [(y*W + x) * 3 + c]  sum_band film * cmf * weight_nm
+ * Synchronous, like pollster::block_on (integrator.rs:127). */
+sky_status sky_render(sky_context *ctx, const sky_planet *planet, const sky_atmosphere *atm,
+                      const sky_wavelengths *wl, const sky_render_config *cfg, float *film,
+                      float *film_m2, float *xyz, sky_render_stats *stats);
+
+/* Same, with device pointers (e.g. torch tensors); ordered on `stream`
+ * (hipStream_t, NULL = the context's stream) and synchronous on return. */
+sky_status sky_render_device(sky_context *ctx, const sky_planet *planet, const sky_atmosphere *atm,
+                             const sky_wavelengths *wl, const sky_render_config *cfg, float *d_film,
+                             float *d_film_m2, float *d_xyz, void *stream, sky_render_stats *stats);
+
+/* ---- deterministic optical depth (Gauss-Legendre per shell) ----
+ * rays: n x 7 doubles (origin xyz km, direction xyz, t_max km);
+ * tau: n x count doubles. */
+sky_status sky_optical_depth(sky_context *ctx, const sky_planet *planet, const sky_atmosphere *atm,
+                             const sky_wavelengths *wl, const double *rays, uint32_t n,
+                             double *tau);
+
+/* Thread-local message of the last error (RenderError's String). */
+const char *sky_last_error(void);
+const char *sky_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

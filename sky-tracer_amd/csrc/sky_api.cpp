@@ -1,0 +1,416 @@
+// sky_api.cpp -- C ABI of libsky_hip.so (declared in include/sky_hip.h).
+//
+// render() of crates/sky-spectral-path-tracer/src/integrator.rs:125-317
+// re-designed for one MI355X: a long-lived context owns the device, a stream
+// and grow-only device buffers (the reference builds a wgpu device, pipeline
+// and buffers on every call, integrator.rs:171-250); one wave-pool launch
+// replaces the 2-spp dispatch loop (integrator.rs:252-283); normalisation and
+// XYZ happen on the device, and only the requested outputs are copied back.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sky_internal.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+sky_status fail(sky_status st, const std::string &msg)
+{
+    g_last_error = msg;
+    return st;
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t bytes)
+    {
+        if (bytes <= n) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipSuccess) n = bytes;
+        return e;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+}  // namespace
+
+struct sky_context {
+    int device = 0;
+    int num_cus = 256;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    sky_tuning tuning{};
+    DevBuf lam, maj, film, film_m2, xyz, accum, counter, diag, rays, tau, lamd;
+};
+
+namespace {
+
+sky_status hip_fail(hipError_t e, const char *what, sky_status st = SKY_ERR_SYNC)
+{
+    if (e == hipErrorOutOfMemory) st = SKY_ERR_OOM;
+    return fail(st, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what, st)                                      \
+    do {                                                             \
+        hipError_t e_ = (expr);                                      \
+        if (e_ != hipSuccess) return hip_fail(e_, what, st);         \
+    } while (0)
+
+// Runs one render into device buffers; outputs already set in hs.dev.
+sky_status run_render(sky_context *ctx, sky::HostScene &hs, hipStream_t stream,
+                      sky_render_stats *stats)
+{
+    sky::DevScene &s = hs.dev;
+    HIP_TRY(ctx->lam.ensure(hs.lam.size() * sizeof(sky::DevLambda)), "alloc lambda table", SKY_ERR_OOM);
+    HIP_TRY(ctx->maj.ensure(hs.maj.size() * sizeof(float)), "alloc majorants", SKY_ERR_OOM);
+    HIP_TRY(ctx->diag.ensure(64), "alloc diagnostics", SKY_ERR_OOM);
+    HIP_TRY(ctx->counter.ensure(64), "alloc work counter", SKY_ERR_OOM);
+    HIP_TRY(hipMemcpyAsync(ctx->lam.p, hs.lam.data(), hs.lam.size() * sizeof(sky::DevLambda),
+                           hipMemcpyHostToDevice, stream),
+            "upload lambda table", SKY_ERR_COPY);
+    HIP_TRY(hipMemcpyAsync(ctx->maj.p, hs.maj.data(), hs.maj.size() * sizeof(float),
+                           hipMemcpyHostToDevice, stream),
+            "upload majorants", SKY_ERR_COPY);
+    HIP_TRY(hipMemsetAsync(ctx->diag.p, 0, 64, stream), "clear diagnostics", SKY_ERR_SYNC);
+    s.lam = (const sky::DevLambda *)ctx->lam.p;
+    s.maj = (const float *)ctx->maj.p;
+    s.diag = (uint32_t *)ctx->diag.p;
+    const bool simple = (s.flags & SKY_FLAG_SIMPLE_KERNEL) != 0;
+    sky::LaunchShape shape;
+    sky::WavePlan plan;
+    if (!simple) {
+        plan = sky::plan_wavepool(s, ctx->tuning, ctx->num_cus);
+        HIP_TRY(ctx->accum.ensure(plan.accum_bytes), "alloc partial sums", SKY_ERR_OOM);
+    } else if (!s.film) {
+        size_t bytes = (size_t)s.width * s.height * s.nlam * sizeof(float);
+        HIP_TRY(ctx->film.ensure(bytes), "alloc film", SKY_ERR_OOM);
+        s.film = (float *)ctx->film.p;
+    }
+    HIP_TRY(hipEventRecord(ctx->ev0, stream), "event record", SKY_ERR_SYNC);
+    if (simple) {
+        HIP_TRY(sky::launch_simple(s, stream, shape), "launch simple integrator", SKY_ERR_SYNC);
+        if (s.xyz) HIP_TRY(sky::launch_film_to_xyz(s, stream), "launch xyz", SKY_ERR_SYNC);
+    } else {
+        HIP_TRY(sky::launch_wavepool(s, plan, ctx->accum.p, (uint32_t *)ctx->counter.p, stream),
+                "launch wave-pool integrator", SKY_ERR_SYNC);
+        shape.grid = plan.grid;
+        shape.block = plan.block;
+        shape.items = plan.n_items;
+    }
+    HIP_TRY(hipEventRecord(ctx->ev1, stream), "event record", SKY_ERR_SYNC);
+    uint32_t wd = 0;
+    HIP_TRY(hipMemcpyAsync(&wd, ctx->diag.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream),
+            "read diagnostics", SKY_ERR_COPY);
+    HIP_TRY(hipStreamSynchronize(stream), "synchronize", SKY_ERR_SYNC);
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    if (stats) {
+        stats->kernel_ms = ms;
+        stats->path_samples = (uint64_t)s.width * s.height * s.spp * s.nlam;
+        stats->watchdog = wd;
+        stats->workgroups = shape.grid;
+        stats->block = shape.block;
+        stats->items = shape.items;
+    }
+    if (wd) return fail(SKY_ERR_WATCHDOG, "GPU path watchdog was reached");  // integrator.rs:301-304
+    return SKY_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *sky_last_error(void) { return g_last_error.c_str(); }
+
+const char *sky_version(void) { return "sky_hip 1 (gfx950 wave-pool integrator)"; }
+
+void sky_default_planet(sky_planet *p)
+{
+    p->ground_radius_km = 6360.0;
+    p->atmosphere_radius_km = 6460.0;
+}
+
+void sky_default_atmosphere(sky_atmosphere *a)
+{
+    a->rayleigh_sca_550_m_inv = 1.3558e-5;
+    a->rayleigh_h_km = 8.0;
+    a->ozone_peak_m_inv = 1.9e-6;
+    a->ozone_center_km = 25.0;
+    a->ozone_half_width_km = 15.0;
+    a->ozone_lambda0_nm = 600.0;
+    a->ozone_sigma_nm = 80.0;
+    a->aerosol_sca_550_m_inv = 3.996e-6;
+    a->aerosol_abs_550_m_inv = 4.4e-6;
+    a->aerosol_h_km = 1.2;
+    a->aerosol_g = 0.8;
+    a->angstrom = 1.3;
+    a->ground_albedo = 0.3;
+    a->sun_temperature_k = 5778.0;
+    a->sun_angular_radius_rad = 0.2675 * 3.14159265358979323846 / 180.0;
+}
+
+// RenderConfig::default (config.rs:38-52) plus the extension defaults.
+void sky_default_config(sky_render_config *c)
+{
+    std::memset(c, 0, sizeof(*c));
+    c->width = 1024;
+    c->height = 512;
+    c->spp = 256;
+    c->seed = 0x5EC720260430ull;
+    c->sun_elevation_deg = 0.0;
+    c->sun_azimuth_deg = 0.0;
+    c->observer_altitude_km = 0.2;
+    c->direct_light_samples = 1;
+    c->projection = SKY_PROJECTION_PANORAMA;
+    c->max_depth = 0;
+    c->rr_start_depth = 4;
+    c->watchdog_limit = 1024;
+    c->max_segments = 100000;
+    c->majorant_layers = 64;
+    c->flags = 0;
+    c->majorant_margin = 1.05;
+}
+
+sky_status sky_context_create(int device, sky_context **out)
+{
+    if (!out) return fail(SKY_ERR_INVALID_CONFIG, "null context pointer");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0)
+        return fail(SKY_ERR_NO_DEVICE, "no usable HIP device");  // RenderError::NoAdapter
+    if (device < 0 || device >= n) return fail(SKY_ERR_NO_DEVICE, "device index out of range");
+    sky_context *c = new sky_context();
+    c->device = device;
+    if ((e = hipSetDevice(device)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess) {
+        delete c;
+        return hip_fail(e, "device initialisation", SKY_ERR_DEVICE_INIT);
+    }
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        c->num_cus = cus;
+    *out = c;
+    return SKY_OK;
+}
+
+void sky_context_destroy(sky_context *c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (DevBuf *b : {&c->lam, &c->maj, &c->film, &c->film_m2, &c->xyz, &c->accum, &c->counter, &c->diag,
+                      &c->rays, &c->tau, &c->lamd})
+        b->release();
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+sky_status sky_context_set_tuning(sky_context *c, const sky_tuning *t)
+{
+    if (!c || !t) return fail(SKY_ERR_INVALID_CONFIG, "null context or tuning");
+    if (t->event_threshold > 64 || t->pixels_per_item > 256 || t->waves_per_cu > 64 || t->track_steps > 4096)
+        return fail(SKY_ERR_INVALID_CONFIG, "tuning value out of range");
+    c->tuning = *t;
+    return SKY_OK;
+}
+
+sky_status sky_context_get_tuning(const sky_context *c, sky_tuning *t)
+{
+    if (!c || !t) return fail(SKY_ERR_INVALID_CONFIG, "null context or tuning");
+    *t = c->tuning;
+    return SKY_OK;
+}
+
+sky_status sky_wavelength_table(const sky_atmosphere *atm, const sky_wavelengths *wl, float *out,
+                                uint32_t n_out)
+{
+    if (!atm || !wl || !out || !wl->center_nm || wl->count == 0 || wl->count > 64)
+        return fail(SKY_ERR_INVALID_CONFIG, "invalid wavelength table request");
+    if (n_out < 8 * wl->count) return fail(SKY_ERR_INVALID_CONFIG, "output too small");
+    std::vector<sky::DevLambda> rows;
+    sky::fill_wavelength_rows(atm, wl, rows);
+    std::memcpy(out, rows.data(), rows.size() * sizeof(sky::DevLambda));
+    return SKY_OK;
+}
+
+sky_status sky_majorant_grid(const sky_planet *planet, const sky_atmosphere *atm,
+                             const sky_wavelengths *wl, uint32_t layers, double margin, float *out,
+                             uint32_t n_out)
+{
+    if (!planet || !atm || !wl || !out || !wl->center_nm || wl->count == 0 || wl->count > 64 ||
+        layers == 0 || layers > 256)
+        return fail(SKY_ERR_INVALID_CONFIG, "invalid majorant grid request");
+    if (n_out < layers * wl->count) return fail(SKY_ERR_INVALID_CONFIG, "output too small");
+    std::vector<float> maj;
+    sky::fill_majorants(planet, atm, wl, layers, margin, maj);
+    std::memcpy(out, maj.data(), maj.size() * sizeof(float));
+    return SKY_OK;
+}
+
+sky_status sky_validate_config(const sky_planet *planet, const sky_atmosphere *atm,
+                               const sky_wavelengths *wl, const sky_render_config *cfg)
+{
+    sky::HostScene hs;
+    std::string err;
+    sky_status st = sky::build_scene(planet, atm, wl, cfg, hs, err);
+    return st == SKY_OK ? SKY_OK : fail(st, err);
+}
+
+sky_status sky_render_device(sky_context *ctx, const sky_planet *planet, const sky_atmosphere *atm,
+                             const sky_wavelengths *wl, const sky_render_config *cfg, float *d_film,
+                             float *d_film_m2, float *d_xyz, void *stream, sky_render_stats *stats)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    // validate before touching the device (integrator.rs:126-127)
+    sky::HostScene hs;
+    std::string err;
+    sky_status st = sky::build_scene(planet, atm, wl, cfg, hs, err);
+    if (st != SKY_OK) return fail(st, err);
+    if (!ctx) return fail(SKY_ERR_NO_DEVICE, "no device context");
+    HIP_TRY(hipSetDevice(ctx->device), "set device", SKY_ERR_DEVICE_INIT);
+    hs.dev.film = d_film;
+    hs.dev.film_m2 = d_film_m2;
+    hs.dev.xyz = d_xyz;
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    st = run_render(ctx, hs, s, stats);
+    if (stats)
+        stats->total_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return st;
+}
+
+sky_status sky_render(sky_context *ctx, const sky_planet *planet, const sky_atmosphere *atm,
+                      const sky_wavelengths *wl, const sky_render_config *cfg, float *film,
+                      float *film_m2, float *xyz, sky_render_stats *stats)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    // validate before touching the device (integrator.rs:126-127)
+    sky::HostScene hs;
+    std::string err;
+    sky_status st = sky::build_scene(planet, atm, wl, cfg, hs, err);
+    if (st != SKY_OK) return fail(st, err);
+    if (!ctx) return fail(SKY_ERR_NO_DEVICE, "no device context");
+    HIP_TRY(hipSetDevice(ctx->device), "set device", SKY_ERR_DEVICE_INIT);
+    const size_t npx = (size_t)cfg->width * cfg->height;
+    const size_t nf = npx * wl->count;
+    if (film) HIP_TRY(ctx->film.ensure(nf * sizeof(float)), "alloc film", SKY_ERR_OOM);
+    if (film_m2) HIP_TRY(ctx->film_m2.ensure(nf * sizeof(float)), "alloc film_m2", SKY_ERR_OOM);
+    if (xyz) HIP_TRY(ctx->xyz.ensure(npx * 3 * sizeof(float)), "alloc xyz", SKY_ERR_OOM);
+    hs.dev.film = film ? (float *)ctx->film.p : nullptr;
+    hs.dev.film_m2 = film_m2 ? (float *)ctx->film_m2.p : nullptr;
+    hs.dev.xyz = xyz ? (float *)ctx->xyz.p : nullptr;
+    st = run_render(ctx, hs, ctx->stream, stats);
+    if (st != SKY_OK && st != SKY_ERR_WATCHDOG) return st;
+    sky_status st_copy = SKY_OK;
+    auto copy = [&](void *dst, const void *src, size_t bytes) {
+        if (!dst || st_copy != SKY_OK) return;
+        hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream);
+        if (e != hipSuccess) st_copy = hip_fail(e, "copy result to host", SKY_ERR_COPY);
+    };
+    copy(film, hs.dev.film, nf * sizeof(float));
+    copy(film_m2, hs.dev.film_m2, nf * sizeof(float));
+    copy(xyz, hs.dev.xyz, npx * 3 * sizeof(float));
+    if (st_copy == SKY_OK) {
+        hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) st_copy = hip_fail(e, "synchronize", SKY_ERR_SYNC);
+    }
+    if (stats)
+        stats->total_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (st_copy != SKY_OK) return st_copy;
+    if (st == SKY_ERR_WATCHDOG) return fail(SKY_ERR_WATCHDOG, "GPU path watchdog was reached");
+    return SKY_OK;
+}
+
+sky_status sky_optical_depth(sky_context *ctx, const sky_planet *planet, const sky_atmosphere *atm,
+                             const sky_wavelengths *wl, const double *rays, uint32_t n, double *tau)
+{
+    if (!ctx) return fail(SKY_ERR_NO_DEVICE, "null context");
+    if (!rays || !tau) return fail(SKY_ERR_INVALID_CONFIG, "null ray or output buffer");
+    sky_render_config cfg;
+    sky_default_config(&cfg);
+    cfg.width = cfg.height = cfg.spp = 1;
+    sky::HostScene hs;
+    std::string err;
+    sky_status st = sky::build_scene(planet, atm, wl, &cfg, hs, err);
+    if (st != SKY_OK) return fail(st, err);
+    if (n == 0) return SKY_OK;
+    for (uint32_t i = 0; i < n; ++i) {
+        const double *r = rays + 7 * (size_t)i;
+        bool ok = true;
+        for (int j = 0; j < 7; ++j) ok = ok && std::isfinite(r[j]);
+        double dn = r[3] * r[3] + r[4] * r[4] + r[5] * r[5];
+        if (!ok || !(dn > 0.0) || !(r[6] >= 0.0 && r[6] <= 1.0e5))
+            return fail(SKY_ERR_INVALID_CONFIG, "rays need finite origins, nonzero directions and 0 <= t_max <= 1e5 km");
+    }
+    HIP_TRY(hipSetDevice(ctx->device), "set device", SKY_ERR_DEVICE_INIT);
+    std::vector<double> lamd(4 * (size_t)wl->count + 4 + 16);
+    for (uint32_t i = 0; i < wl->count; ++i) {
+        // FP64 coefficients for the quadrature (same formulas as the scene build)
+        double l = wl->center_nm[i];
+        double r = 550.0 / l, dl = (l - atm->ozone_lambda0_nm) / atm->ozone_sigma_nm;
+        double ang = std::pow(l / 550.0, -atm->angstrom);
+        lamd[4 * i + 0] = atm->rayleigh_sca_550_m_inv * 1000.0 * r * r * r * r;
+        lamd[4 * i + 1] = atm->ozone_peak_m_inv * 1000.0 * std::exp(-0.5 * dl * dl);
+        lamd[4 * i + 2] = atm->aerosol_sca_550_m_inv * 1000.0 * ang;
+        lamd[4 * i + 3] = atm->aerosol_abs_550_m_inv * 1000.0 * ang;
+    }
+    lamd[4 * wl->count + 0] = atm->rayleigh_h_km;
+    lamd[4 * wl->count + 1] = atm->aerosol_h_km;
+    lamd[4 * wl->count + 2] = atm->ozone_center_km;
+    lamd[4 * wl->count + 3] = atm->ozone_half_width_km;
+    // 16-point Gauss-Legendre rule by Newton iteration on P_16
+    for (int i = 0; i < 8; ++i) {
+        double x = std::cos(3.14159265358979323846 * (i + 0.75) / 16.5), dp = 0.0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = x;
+            for (int k = 2; k <= 16; ++k) {
+                double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = 16.0 * (x * p1 - p0) / (x * x - 1.0);
+            double dx = p1 / dp;
+            x -= dx;
+            if (std::fabs(dx) < 1e-16) break;
+        }
+        lamd[4 * wl->count + 4 + (7 - i)] = x;
+        lamd[4 * wl->count + 12 + (7 - i)] = 2.0 / ((1.0 - x * x) * dp * dp);
+    }
+    size_t rb = (size_t)n * 7 * sizeof(double), tb = (size_t)n * wl->count * sizeof(double);
+    HIP_TRY(ctx->rays.ensure(rb), "alloc rays", SKY_ERR_OOM);
+    HIP_TRY(ctx->tau.ensure(tb), "alloc tau", SKY_ERR_OOM);
+    HIP_TRY(ctx->lamd.ensure(lamd.size() * sizeof(double)), "alloc coefficients", SKY_ERR_OOM);
+    HIP_TRY(hipMemcpyAsync(ctx->rays.p, rays, rb, hipMemcpyHostToDevice, ctx->stream), "upload rays", SKY_ERR_COPY);
+    HIP_TRY(hipMemcpyAsync(ctx->lamd.p, lamd.data(), lamd.size() * sizeof(double), hipMemcpyHostToDevice,
+                           ctx->stream),
+            "upload coefficients", SKY_ERR_COPY);
+    HIP_TRY(sky::launch_optical_depth(hs.dev, (const double *)ctx->rays.p, n, (const double *)ctx->lamd.p,
+                                      (double *)ctx->tau.p, ctx->stream),
+            "launch optical depth", SKY_ERR_SYNC);
+    HIP_TRY(hipMemcpyAsync(tau, ctx->tau.p, tb, hipMemcpyDeviceToHost, ctx->stream), "copy tau", SKY_ERR_COPY);
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "synchronize", SKY_ERR_SYNC);
+    return SKY_OK;
+}
+
+}  // extern "C"

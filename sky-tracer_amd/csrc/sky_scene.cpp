@@ -1,0 +1,261 @@
+// sky_scene.cpp -- host-side scene build for the MI355X integrator.
+//
+// Restates the scene assembly of crates/sky-core/src/data.rs:15-51 and the
+// GPU packing of crates/sky-spectral-path-tracer/src/integrator.rs:319-418 for
+// the analytic atmosphere: per-wavelength coefficients, the Planck sun, the
+// CIE weights, the per-shell majorant grid (medium.rs:63-83) and the constant
+// block the kernels read.
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "sky_internal.hpp"
+
+namespace sky {
+
+namespace {
+
+constexpr double kPiD = 3.14159265358979323846;
+
+// One lobe of the Wyman-Sloan-Shirley multi-lobe fit of the CIE 1931 2-degree
+// colour-matching functions (JCGT 2(2), 2013).  Replaces the table lerp of
+// spectrum.rs:196-219.
+double wss_lobe(double l, double mu, double s_lo, double s_hi)
+{
+    double t = (l - mu) / (l < mu ? s_lo : s_hi);
+    return std::exp(-0.5 * t * t);
+}
+
+void cmf_xyz(double l, double out[3])
+{
+    out[0] = 1.056 * wss_lobe(l, 599.8, 37.9, 31.0) + 0.362 * wss_lobe(l, 442.0, 16.0, 26.7) -
+             0.065 * wss_lobe(l, 501.1, 20.4, 26.2);
+    out[1] = 0.821 * wss_lobe(l, 568.8, 46.9, 40.5) + 0.286 * wss_lobe(l, 530.9, 16.3, 31.1);
+    out[2] = 1.217 * wss_lobe(l, 437.0, 11.8, 36.0) + 0.681 * wss_lobe(l, 459.0, 26.0, 13.8);
+}
+
+// Planck B_lambda(T), W m^-2 sr^-1 nm^-1 (replaces E_band / Omega_sun of
+// data.rs:30-33).
+double planck(double lambda_nm, double temperature_k)
+{
+    const double h = 6.62607015e-34, c = 299792458.0, k = 1.380649e-23;
+    double l = lambda_nm * 1e-9;
+    return 2.0 * h * c * c / (l * l * l * l * l) / std::expm1(h * c / (l * k * temperature_k)) * 1e-9;
+}
+
+double default_weight(const sky_wavelengths *wl)
+{
+    if (wl->count < 2) return 1.0;
+    return (wl->center_nm[wl->count - 1] - wl->center_nm[0]) / (wl->count - 1);
+}
+
+}  // namespace
+
+// km^-1 coefficients at the profiles' reference heights: Rayleigh
+// beta (550/l)^4, ozone Gaussian band, aerosol Angstrom law (replaces the
+// cross-section tables of data.rs:26-29 and medium.rs:104-106).
+static void coeffs_d(const sky_atmosphere *a, double l, double c[4])
+{
+    double r = 550.0 / l;
+    double dl = (l - a->ozone_lambda0_nm) / a->ozone_sigma_nm;
+    double ang = std::pow(l / 550.0, -a->angstrom);
+    c[0] = a->rayleigh_sca_550_m_inv * 1000.0 * r * r * r * r;
+    c[1] = a->ozone_peak_m_inv * 1000.0 * std::exp(-0.5 * dl * dl);
+    c[2] = a->aerosol_sca_550_m_inv * 1000.0 * ang;
+    c[3] = a->aerosol_abs_550_m_inv * 1000.0 * ang;
+}
+
+void fill_wavelength_rows(const sky_atmosphere *a, const sky_wavelengths *wl,
+                          std::vector<DevLambda> &rows)
+{
+    rows.resize(wl->count);
+    double dw = default_weight(wl);
+    for (uint32_t i = 0; i < wl->count; ++i) {
+        double l = wl->center_nm[i];
+        double w = wl->weight_nm ? wl->weight_nm[i] : dw;
+        double cmf[3], c[4];
+        cmf_xyz(l, cmf);
+        coeffs_d(a, l, c);
+        DevLambda &d = rows[i];
+        d.br = (float)c[0];
+        d.bo = (float)c[1];
+        d.bas = (float)c[2];
+        d.baa = (float)c[3];
+        d.lsun = (float)planck(l, a->sun_temperature_k);
+        d.wx = (float)(cmf[0] * w);
+        d.wy = (float)(cmf[1] * w);
+        d.wz = (float)(cmf[2] * w);
+    }
+}
+
+// margin * max over shell [k dz, (k+1) dz] of sigma_t, floored at 1e-8
+// (medium.rs:77, sky_trace.wgsl:260).  Exact for these profiles: the
+// exponentials peak at the shell's lower edge, the tent at the point of the
+// shell nearest its centre.
+void fill_majorants(const sky_planet *planet, const sky_atmosphere *a, const sky_wavelengths *wl,
+                    uint32_t layers, double margin, std::vector<float> &out)
+{
+    double top = planet->atmosphere_radius_km - planet->ground_radius_km;
+    double dz = top / layers;
+    out.resize((size_t)wl->count * layers);
+    for (size_t b = 0; b < wl->count; ++b) {
+        double c[4];
+        coeffs_d(a, wl->center_nm[b], c);
+        for (uint32_t k = 0; k < layers; ++k) {
+            double lo = k * dz, hi = (k + 1) * dz;
+            double d = 0.0;
+            if (a->ozone_center_km < lo) d = lo - a->ozone_center_km;
+            if (a->ozone_center_km > hi) d = a->ozone_center_km - hi;
+            double tent = std::fmax(1.0 - d / a->ozone_half_width_km, 0.0);
+            double m = c[0] * std::exp(-lo / a->rayleigh_h_km) +
+                       (c[2] + c[3]) * std::exp(-lo / a->aerosol_h_km) + c[1] * tent;
+            m *= margin;
+            out[b * layers + k] = (float)std::fmax(m, 1e-8);
+        }
+    }
+}
+
+static bool finite_pos(double v) { return std::isfinite(v) && v > 0.0; }
+
+sky_status build_scene(const sky_planet *planet, const sky_atmosphere *a, const sky_wavelengths *wl,
+                       const sky_render_config *cfg, HostScene &out, std::string &err)
+{
+    // validate_config, integrator.rs:130-163 (bands generalised to 1..64)
+    if (!planet || !a || !wl || !cfg) {
+        err = "null scene or config pointer";
+        return SKY_ERR_INVALID_CONFIG;
+    }
+    if (wl->count == 0 || wl->count > (uint32_t)kMaxLambda || !wl->center_nm) {
+        err = "expected 1..64 wavelengths, found " + std::to_string(wl->count);
+        return SKY_ERR_INVALID_CONFIG;
+    }
+    for (uint32_t i = 0; i < wl->count; ++i) {
+        if (!finite_pos(wl->center_nm[i]) || (wl->weight_nm && !std::isfinite(wl->weight_nm[i]))) {
+            err = "wavelengths and weights must be finite and positive";
+            return SKY_ERR_INVALID_CONFIG;
+        }
+    }
+    if (cfg->width == 0 || cfg->height == 0 || cfg->spp == 0) {
+        err = "width, height, and spp must be greater than zero";
+        return SKY_ERR_INVALID_CONFIG;
+    }
+    uint64_t cells = (uint64_t)cfg->width * cfg->height * wl->count;
+    if (cells > 0xFFFFFFFFull / 2) {
+        err = "film is too large for the GPU shader index type";
+        return SKY_ERR_INVALID_CONFIG;
+    }
+    if ((uint64_t)cfg->sample_offset + cfg->spp > 0xFFFFFFFFull) {
+        err = "sample range overflows the 32-bit sample key";
+        return SKY_ERR_INVALID_CONFIG;
+    }
+    if (cfg->majorant_layers == 0 || cfg->majorant_layers > (uint32_t)kMaxLayers) {
+        err = "majorant grid must contain 1..256 layers";
+        return SKY_ERR_INVALID_CONFIG;
+    }
+    if (!finite_pos(planet->ground_radius_km) ||
+        !(planet->atmosphere_radius_km > planet->ground_radius_km) ||
+        !std::isfinite(planet->atmosphere_radius_km)) {
+        err = "planet radii must satisfy 0 < ground < atmosphere";
+        return SKY_ERR_INVALID_CONFIG;
+    }
+    if (!finite_pos(a->rayleigh_h_km) || !finite_pos(a->aerosol_h_km) ||
+        !finite_pos(a->ozone_half_width_km) || !finite_pos(a->ozone_sigma_nm) ||
+        !finite_pos(a->sun_temperature_k) || !finite_pos(a->sun_angular_radius_rad) ||
+        !(a->sun_angular_radius_rad < 0.5 * kPiD)) {
+        err = "scale heights, widths, sun temperature and radius must be positive";
+        return SKY_ERR_INVALID_CONFIG;
+    }
+    if (!(a->aerosol_g > -1.0 && a->aerosol_g < 1.0)) {
+        err = "aerosol asymmetry g must lie in (-1, 1)";
+        return SKY_ERR_INVALID_CONFIG;
+    }
+    if (!(a->ground_albedo >= 0.0 && a->ground_albedo <= 1.0)) {
+        err = "ground albedo must lie in [0, 1]";
+        return SKY_ERR_INVALID_CONFIG;
+    }
+    if (!(a->rayleigh_sca_550_m_inv >= 0.0) || !(a->ozone_peak_m_inv >= 0.0) ||
+        !(a->aerosol_sca_550_m_inv >= 0.0) || !(a->aerosol_abs_550_m_inv >= 0.0) ||
+        !std::isfinite(a->angstrom) || !std::isfinite(a->ozone_center_km)) {
+        err = "medium coefficients must be finite and non-negative";
+        return SKY_ERR_INVALID_CONFIG;
+    }
+    if (!(cfg->majorant_margin >= 1.0) || !std::isfinite(cfg->majorant_margin)) {
+        err = "majorant margin must be >= 1";
+        return SKY_ERR_INVALID_CONFIG;
+    }
+    if (cfg->projection > 1u) {
+        err = "unknown output projection";
+        return SKY_ERR_INVALID_CONFIG;
+    }
+
+    out.nlam = wl->count;
+    fill_wavelength_rows(a, wl, out.lam);
+    fill_majorants(planet, a, wl, cfg->majorant_layers, cfg->majorant_margin, out.maj);
+
+    DevScene &s = out.dev;
+    std::memset(&s, 0, sizeof(s));
+    s.R = planet->ground_radius_km;
+    s.top = planet->atmosphere_radius_km - planet->ground_radius_km;
+    s.L = cfg->majorant_layers;
+    s.dz = s.top / s.L;
+    s.delta_top = (planet->atmosphere_radius_km - s.R) * (planet->atmosphere_radius_km + s.R);
+    s.obs_y = s.R + std::fmax(cfg->observer_altitude_km, 0.0);
+    s.Rf = (float)s.R;
+    s.R2f = (float)(s.R * s.R);
+    s.topf = (float)s.top;
+    s.eps = kRayEpsilonKm;
+    // Sun::from_degrees, atmosphere.rs:20-29
+    double e = cfg->sun_elevation_deg * kPiD / 180.0, az = cfg->sun_azimuth_deg * kPiD / 180.0;
+    double sd[3] = {std::cos(e) * std::sin(az), std::sin(e), std::cos(e) * std::cos(az)};
+    double sn = std::sqrt(sd[0] * sd[0] + sd[1] * sd[1] + sd[2] * sd[2]);
+    for (int i = 0; i < 3; ++i) s.sun[i] = (float)(sd[i] / sn);
+    double hsin = std::sin(0.5 * a->sun_angular_radius_rad);
+    double omc = 2.0 * hsin * hsin;
+    s.omc = (float)omc;
+    s.cone_pdf = (float)(1.0 / (2.0 * kPiD * omc));
+    s.inv_hr = (float)(1.0 / a->rayleigh_h_km);
+    s.inv_ha = (float)(1.0 / a->aerosol_h_km);
+    s.hc = (float)a->ozone_center_km;
+    s.inv_hw = (float)(1.0 / a->ozone_half_width_km);
+    s.g = (float)a->aerosol_g;
+    s.albedo = (float)a->ground_albedo;
+    // Sky-view LUT frame, sky_trace.wgsl:632-681
+    {
+        double vh = s.obs_y, R = s.R;
+        double v_horizon = std::sqrt(std::fmax(vh * vh - R * R, 0.0));
+        double cos_beta = std::fmin(std::fmax(v_horizon / std::fmax(vh, 1e-6), 0.0), 1.0);
+        double beta = std::acos(cos_beta);
+        s.svl_zha = (float)(kPiD - beta);
+        s.svl_beta = (float)beta;
+        double sh[3] = {sd[0] / sn, 0.0, sd[2] / sn};
+        double shl = std::sqrt(sh[0] * sh[0] + sh[2] * sh[2]);
+        double fw[3] = {1.0, 0.0, 0.0};
+        if (shl > 1e-5) {
+            fw[0] = sh[0] / shl;
+            fw[2] = sh[2] / shl;
+        }
+        double side[3] = {fw[2], 0.0, -fw[0]};
+        double sl = std::sqrt(side[0] * side[0] + side[2] * side[2]);
+        for (int i = 0; i < 3; ++i) {
+            s.svl_fw[i] = (float)fw[i];
+            s.svl_side[i] = (float)(side[i] / sl);
+        }
+    }
+    s.width = cfg->width;
+    s.height = cfg->height;
+    s.spp = cfg->spp;
+    s.sample_offset = cfg->sample_offset;
+    s.seed_lo = (uint32_t)cfg->seed;
+    s.seed_hi = (uint32_t)(cfg->seed >> 32);
+    s.nlam = wl->count;
+    s.dls = cfg->direct_light_samples ? cfg->direct_light_samples : 1u;  // integrator.rs:325
+    s.projection = cfg->projection;
+    s.max_depth = cfg->max_depth;
+    s.rr_start = cfg->rr_start_depth;
+    s.watchdog_limit = cfg->watchdog_limit;
+    s.max_segments = cfg->max_segments;
+    s.flags = cfg->flags;
+    return SKY_OK;
+}
+
+}  // namespace sky

@@ -1,0 +1,30 @@
+"""pysky -- Python host of the MI355X spectral sky path tracer (libsky_hip.so).
+
+Mirrors the reference's integrator interface (render(scene, config) -> Film,
+RenderError variants) over the C ABI of include/sky_hip.h.
+"""
+from ._lib import LIB_PATH, HEADER_PATH, lib  # noqa: F401
+from .api import (  # noqa: F401
+    PARAM_NAMES,
+    AtmosphereParams,
+    BufferMap,
+    Context,
+    Film,
+    InvalidConfig,
+    NoAdapter,
+    OutOfMemory,
+    OutputProjection,
+    Poll,
+    RenderConfig,
+    RenderError,
+    RequestDevice,
+    SceneData,
+    Watchdog,
+    default_context,
+    load_scene,
+    majorant_grid,
+    midpoint_grid,
+    optical_depth,
+    render,
+    wavelength_table,
+)

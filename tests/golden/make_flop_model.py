@@ -1,0 +1,88 @@
+"""Freezes the algorithmic-flop model used by bench.py's roofline.
+
+F_unit(config, mode) = sum_e  n_e(config, mode) * c_e, where n_e is the mean
+count per path x lambda sample of event e measured by the CPU oracle at the
+configuration's seed on its subsampled pixel grid, and c_e the fixed FP32 cost
+of the event's arithmetic (FMA = 2, add/mul/cmp/min/max/cvt = 1, each
+transcendental or sqrt/div = 1).  pcg4d integer work is reported separately.
+
+Run from the repository root:  python tests/golden/make_flop_model.py
+Writes sky-tracer_amd/pysky/flop_model.json.
+"""
+import json
+import os
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import numpy as np  # noqa: E402
+
+import oracle as O  # noqa: E402
+
+# FP32 operations per event of the integrator's arithmetic.
+COSTS = {
+    "paths": 26,             # jitter, equirect direction (2 sincos), normalise, XYZ weight
+    "boundary_queries": 40,  # ray set-up: b, q, top/ground roots, initial shell
+    "segments": 11,          # shell crossing: D_k - q, root, segment optical depth
+    "free_flights": 6,       # -log(1 - u), distance
+    "density_evals": 26,     # altitude from (b, q), two exponentials, tent, ratio
+    "scatter_vertices": 110, # FP64 position, coefficients, albedo, phase sample, frame
+    "nee_samples": 80,       # sun cone sample, frame, phase or cosine factor
+    "ground_vertices": 85,   # position, normal, cosine sample, frame, roulette
+}
+# pcg4d per draw with the per-path words pre-multiplied: 8 mul, 8 add,
+# 4 shift + 4 xor, 3 xor fold, shift, convert, scale.
+INT_OPS_PER_DRAW = 31
+
+WORKLOADS = {
+    # name: (width, height, n_lambda, spp_subsample, max_depth, sun, albedo, aerosol_scale, grid_step)
+    "c1": (128, 64, 16, 16, 8, 10.0, 0.3, 1.0, 1),
+    "c2": (1024, 512, 64, 16, 32, 5.0, 0.3, 1.0, 16),
+    "c3": (512, 256, 64, 8, 64, -4.0, 0.8, 4.0, 8),
+}
+MODES = {
+    "reference": 0,
+    "carried": O.FLAG_CARRIED_TAU,
+    "carried_is": O.FLAG_CARRIED_TAU | O.FLAG_PHASE_IS,
+    "carried_is_ratio": O.FLAG_CARRIED_TAU | O.FLAG_PHASE_IS | O.FLAG_RATIO_TRACKING,
+}
+
+
+def measure(workload, mode, layers):
+    w, h, nl, spp, depth, sun, alb, aer, step = WORKLOADS[workload]
+    p = O.Params(width=w, height=h, spp=spp, seed=1, sun_elevation_deg=sun, max_depth=depth,
+                 ground_albedo=alb, aerosol_sca_550_m_inv=3.996e-6 * aer,
+                 aerosol_abs_550_m_inv=4.4e-6 * aer, lambda_nm=list(O.midpoint_grid(nl)),
+                 flags=MODES[mode], majorant_layers=layers)
+    xs = np.arange(step // 2, w, step)
+    ys = np.arange(step // 2, h, step)
+    pixels = (ys[:, None] * w + xs[None, :]).ravel()
+    _, _, cnt, rc = O.render_pixels(p, pixels, "f32")
+    assert rc == 0
+    n = cnt["paths"]
+    ev = {k: cnt[k] / n for k in cnt if k != "paths"}
+    flops = COSTS["paths"] + sum(COSTS[k] * ev.get(k, 0.0) for k in COSTS if k != "paths")
+    return {
+        "events_per_sample": ev,
+        "flops_per_sample": flops,
+        "rng_int_ops_per_sample": INT_OPS_PER_DRAW * ev["rng_draws"],
+        "sample": f"{len(pixels)} px (every {step}th) x {nl} lambda x {spp} spp, seed 1",
+    }
+
+
+def main():
+    out = {"costs": COSTS, "int_ops_per_rng_draw": INT_OPS_PER_DRAW, "configs": {}}
+    for wl in ("c1", "c2", "c3"):
+        for mode in ("reference", "carried", "carried_is"):
+            for layers in (16, 64):
+                key = f"{wl}:{mode}:L{layers}"
+                out["configs"][key] = measure(wl, mode, layers)
+                print(key, round(out["configs"][key]["flops_per_sample"], 1), flush=True)
+    dst = os.path.join(ROOT, "sky-tracer_amd", "pysky", "flop_model.json")
+    with open(dst, "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+    print("wrote", dst)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,136 @@
+"""C-ABI checks that need no GPU: the library loads, exports every function
+include/sky_hip.h declares, validates like integrator.rs:130-163, and its host
+scene build (per-wavelength table, majorant grid) matches the oracle."""
+import ctypes as C
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+import pysky
+from pysky import _lib as L
+
+
+def _declared_functions():
+    text = open(L.HEADER_PATH).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[a-z_]+\s*\**\s*(sky_[a-z0-9_]+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(L.LIB_PATH)
+    names = _declared_functions()
+    assert len(names) >= 14
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_struct_sizes_match_header_layout():
+    # the Python mirrors must have the C layout (no padding surprises)
+    assert C.sizeof(L.RenderConfigC) == 4 * 4 + 8 + 3 * 8 + 8 * 4 + 8
+    assert C.sizeof(L.Atmosphere) == 15 * 8
+    assert C.sizeof(L.Tuning) == 4 * 4
+
+
+def test_defaults_match_reference():
+    cfg = L.RenderConfigC()
+    L.lib().sky_default_config(C.byref(cfg))
+    # config.rs:38-52
+    assert (cfg.width, cfg.height, cfg.spp) == (1024, 512, 256)
+    assert cfg.seed == 0x5EC7_2026_0430
+    assert cfg.observer_altitude_km == pytest.approx(0.2)
+    assert cfg.direct_light_samples == 1 and cfg.projection == 0
+    # integrator.rs:14, sky_trace.wgsl:412/692, data.rs:48, medium.rs:77
+    assert cfg.watchdog_limit == 1024 and cfg.max_segments == 100000
+    assert cfg.rr_start_depth == 4 and cfg.majorant_layers == 64
+    assert cfg.majorant_margin == pytest.approx(1.05)
+    py = pysky.RenderConfig()
+    assert (py.width, py.height, py.spp, py.seed) == (1024, 512, 256, 0x5EC7_2026_0430)
+
+
+@pytest.mark.parametrize("n", [3, 16, 41, 64])
+def test_wavelength_table_matches_oracle(n):
+    lam = list(np.sort(np.random.default_rng(n).uniform(390, 770, n))) if n in (3, 41) else list(pysky.midpoint_grid(n))
+    scene = pysky.SceneData(wavelengths_nm=lam)
+    got = pysky.wavelength_table(scene)
+    ref = O.lambda_table(O.Params(lambda_nm=lam))
+    np.testing.assert_allclose(got, ref, rtol=2e-7)
+
+
+@pytest.mark.parametrize("layers", [8, 16, 64, 200])
+def test_majorant_grid_matches_oracle(layers):
+    scene = pysky.SceneData(wavelengths_nm=list(pysky.midpoint_grid(16)))
+    got = pysky.majorant_grid(scene, layers)
+    ref = O.majorant_grid(O.Params(majorant_layers=layers))
+    np.testing.assert_allclose(got, ref, rtol=2e-7)
+
+
+@pytest.mark.parametrize("bad", [
+    dict(width=0), dict(height=0), dict(spp=0), dict(majorant_layers=0), dict(majorant_layers=300),
+    dict(majorant_margin=0.9), dict(output_projection=2), dict(width=70000, height=70000),
+])
+def test_invalid_config_is_reported_before_device(bad):
+    scene = pysky.load_scene(10.0)
+    cfg = pysky.RenderConfig(width=4, height=2, spp=1)
+    for k, v in bad.items():
+        setattr(cfg, k, v)
+    with pytest.raises(pysky.InvalidConfig):
+        pysky.render(scene, cfg, context=_null_context())
+
+
+@pytest.mark.parametrize("bad", [dict(wavelengths_nm=[]), dict(wavelengths_nm=[500.0] * 65),
+                                 dict(wavelengths_nm=[-1.0])])
+def test_invalid_wavelengths(bad):
+    scene = pysky.load_scene(10.0, **{})
+    scene.wavelengths_nm = bad["wavelengths_nm"]
+    with pytest.raises(pysky.InvalidConfig):
+        pysky.render(scene, pysky.RenderConfig(width=4, height=2, spp=1), context=_null_context())
+
+
+@pytest.mark.parametrize("field,value", [("aerosol_g", 1.0), ("ground_albedo", 1.5), ("rayleigh_h_km", 0.0),
+                                         ("sun_angular_radius_rad", 0.0)])
+def test_invalid_atmosphere(field, value):
+    scene = pysky.load_scene(10.0)
+    setattr(scene.atmosphere, field, value)
+    with pytest.raises(pysky.InvalidConfig):
+        pysky.render(scene, pysky.RenderConfig(width=4, height=2, spp=1), context=_null_context())
+
+
+class _NullCtx:
+    handle = None
+
+
+def _null_context():
+    return _NullCtx()
+
+
+def test_valid_config_without_device_is_no_adapter():
+    """A valid render with no device context maps to RenderError::NoAdapter
+    (integrator.rs:179), which smoke.rs:18 treats as a skip."""
+    with pytest.raises(pysky.NoAdapter):
+        pysky.render(pysky.load_scene(5.0), pysky.RenderConfig(width=4, height=2, spp=1), context=_null_context())
+
+
+def test_context_create_status():
+    h = C.c_void_p()
+    st = L.lib().sky_context_create(0, C.byref(h))
+    assert st in (L.SKY_OK, L.SKY_ERR_NO_DEVICE)
+    if st == L.SKY_OK:
+        L.lib().sky_context_destroy(h)
+    st = L.lib().sky_context_create(-1, C.byref(h))
+    assert st == L.SKY_ERR_NO_DEVICE
+
+
+def test_film_api_mirrors_reference():
+    """film.rs:2-62."""
+    v = np.arange(2 * 3 * 4, dtype=np.float32)
+    f = pysky.Film(3, 2, 4, v.copy())
+    assert (f.width(), f.height(), f.band_count()) == (3, 2, 4)
+    np.testing.assert_array_equal(f.pixel_spectrum(1), [4, 5, 6, 7])
+    f.set_pixel_spectrum(5, [9, 9, 9, 9])
+    np.testing.assert_array_equal(f.values()[20:24], [9, 9, 9, 9])
+    assert f.as_array().shape == (2, 3, 4)
+    assert pysky.OutputProjection.Panorama.label() == "panorama"
+    assert pysky.OutputProjection.SkyViewLut.label() == "sky-view LUT"

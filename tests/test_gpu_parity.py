@@ -1,0 +1,151 @@
+"""GPU parity of the HIP integrator against the CPU oracle.
+
+Every path is keyed (pixel, sample, band, dim) exactly as the reference keys
+it (sky_trace.wgsl:170-189), so at 1 spp each film entry is one path and the
+GPU and the FP32 oracle trace the same random walk: they agree path by path
+up to float rounding (1-ulp differences in log/exp/sqrt can flip a decision
+on a small fraction of paths).  Means over many samples are compared with
+per-cell z-scores from the second-moment film.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import pysky
+from helpers import case, paired_stats, zscores
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu(scene, cfg, ctx, want_m2=False):
+    print(f"[gpu] render {cfg.width}x{cfg.height} nl={scene.band_count} spp={cfg.spp} flags={cfg.flags} "
+          f"layers={cfg.majorant_layers}", flush=True)
+    return pysky.render(scene, cfg, want_m2=want_m2, context=ctx)
+
+
+@pytest.mark.parametrize("mode", ["reference", "carried", "carried_is", "carried_is_ratio"])
+@pytest.mark.parametrize("simple", [False, True])
+def test_paths_pair_with_oracle(gpu_context, mode, simple):
+    scene, cfg, op = case(width=64, height=32, spp=1, mode=mode, simple_kernel=simple, max_depth=8)
+    film = _gpu(scene, cfg, gpu_context).values()
+    ref, _, _, _, rc = O.render(op, "f32")
+    assert rc == 0
+    frac, bad = paired_stats(film, ref.ravel(), rtol=1e-3)
+    print(f"paired fraction {frac:.5f} ({len(bad)} of {film.size} differ)")
+    assert frac >= 0.995
+    assert np.all(np.isfinite(film)) and np.all(film >= 0)
+
+
+@pytest.mark.parametrize("mode", ["carried", "carried_is"])
+def test_sample_offset_pairs(gpu_context, mode):
+    # a later sample index exercises other RNG keys (seed_hi ^ sample)
+    scene, cfg, op = case(width=32, height=16, spp=1, sample_offset=977, mode=mode, seed=0x5EC720260430)
+    film = _gpu(scene, cfg, gpu_context).values()
+    ref, _, _, _, _ = O.render(op, "f32")
+    frac, _ = paired_stats(film, ref.ravel())
+    assert frac >= 0.995
+
+
+def test_means_match_reference_estimator(gpu_context):
+    """GPU (carried optical depth, phase IS) vs the oracle's reference-form
+    estimator (per-segment resampling, isotropic phase): independent random
+    streams for the same expectation, 64 spp on the 128x64x16 case."""
+    scene, cfg, _ = case(spp=64, mode="carried_is")
+    g = _gpu(scene, cfg, gpu_context, want_m2=True)
+    _, _, op = case(spp=64, mode="reference", seed=2)
+    rf, rm2, _, _, rc = O.render(op, "f32", want_m2=True)
+    assert rc == 0
+    gf = g.values().reshape(rf.shape).astype(np.float64)
+    gm2 = g.m2.reshape(rf.shape).astype(np.float64)
+    z, mask = zscores(gf, gm2, 64, rf, rm2, 64)
+    zz = z[mask]
+    print(f"z mean {zz.mean():.4f} std {zz.std():.4f} frac>3 {(np.abs(zz) > 3).mean():.5f} n {zz.size}")
+    assert abs(zz.mean()) < 0.03
+    assert 0.9 < zz.std() < 1.1
+    assert (np.abs(zz) > 4).mean() < 2e-3
+
+
+def test_xyz_is_weighted_film(gpu_context):
+    scene, cfg, op = case(width=64, height=32, spp=8, mode="carried")
+    g = _gpu(scene, cfg, gpu_context)
+    lam = O.lambda_table(op)
+    film = g.as_array().astype(np.float64)
+    xyz = film @ lam[:, 5:8]
+    np.testing.assert_allclose(g.xyz.reshape(xyz.shape), xyz, rtol=2e-5, atol=1e-6 * xyz.max())
+
+
+def test_sample_shards_sum_to_full_run(gpu_context):
+    scene, cfg, _ = case(width=64, height=32, spp=64, mode="carried")
+    full = _gpu(scene, cfg, gpu_context).values().astype(np.float64)
+    parts = []
+    for k in range(4):
+        cfg.spp, cfg.sample_offset = 16, 16 * k
+        parts.append(_gpu(scene, cfg, gpu_context).values().astype(np.float64))
+    np.testing.assert_allclose(np.mean(parts, axis=0), full, rtol=1e-4, atol=1e-6 * full.max())
+
+
+def test_wavepool_pairs_with_simple_kernel(gpu_context):
+    scene, cfg, _ = case(width=64, height=32, spp=1, mode="carried_is")
+    a = _gpu(scene, cfg, gpu_context).values()
+    cfg.simple_kernel = True
+    b = _gpu(scene, cfg, gpu_context).values()
+    frac, _ = paired_stats(a, b)
+    assert frac >= 0.998
+
+
+def test_deterministic(gpu_context):
+    scene, cfg, _ = case(width=64, height=32, spp=8, mode="carried")
+    a = _gpu(scene, cfg, gpu_context).values()
+    b = _gpu(scene, cfg, gpu_context).values()
+    assert np.array_equal(a, b)
+
+
+def test_sky_view_lut_pairs(gpu_context):
+    scene, cfg, op = case(width=32, height=32, spp=1, projection=1, mode="carried")
+    film = _gpu(scene, cfg, gpu_context).values()
+    ref, _, _, _, _ = O.render(op, "f32")
+    frac, _ = paired_stats(film, ref.ravel())
+    assert frac >= 0.99
+
+
+def test_reference_smoke(gpu_context):
+    """smoke.rs:7-24: 32x16 at 2 spp, all finite and >= 0, some > 0."""
+    scene = pysky.load_scene(0.0, 0.0)
+    film = pysky.render(scene, pysky.RenderConfig(width=32, height=16, spp=2), context=gpu_context)
+    v = film.values()
+    assert np.all(np.isfinite(v)) and np.all(v >= 0) and np.any(v > 0)
+
+
+def test_watchdog_depth_limit(gpu_context):
+    scene, cfg, _ = case(width=16, height=8, spp=2, max_depth=0, watchdog_limit=0)
+    with pytest.raises(pysky.Watchdog):
+        _gpu(scene, cfg, gpu_context)
+
+
+def test_watchdog_segment_guard(gpu_context):
+    scene, cfg, _ = case(width=16, height=8, spp=2, max_segments=0)
+    with pytest.raises(pysky.Watchdog):
+        _gpu(scene, cfg, gpu_context)
+
+
+def test_invalid_config(gpu_context):
+    scene, cfg, _ = case(width=16, height=8, spp=0)
+    with pytest.raises(pysky.InvalidConfig):
+        _gpu(scene, cfg, gpu_context)
+
+
+def test_optical_depth_matches_oracle(gpu_context):
+    rng = np.random.default_rng(3)
+    scene, _, op = case(n_lambda=16)
+    rays = []
+    for _ in range(64):
+        h = rng.uniform(0.0, 99.0)
+        o = np.array([0.0, 6360.0 + h, 0.0])
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        hit, t, kind = O.next_boundary_reference(op, o, d)
+        rays.append(np.concatenate([o, d, [t]]))
+    rays = np.array(rays)
+    tau = pysky.optical_depth(scene, rays, context=gpu_context)
+    ref = np.array([[O.optical_depth(op, b, r[:3], r[3:6], r[6]) for b in range(16)] for r in rays])
+    np.testing.assert_allclose(tau, ref, rtol=1e-9, atol=1e-14)

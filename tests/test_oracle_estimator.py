@@ -1,0 +1,179 @@
+"""Statistical known-answer tests of the oracle's estimator (CPU only).
+
+  * delta/ratio tracking: E[T] = exp(-tau) with tau from the pinned quadrature;
+  * vacuum: a pixel containing the sun averages L_sun x P(jittered ray in disk);
+  * bare Lambertian ground: L = albedo/pi * L_sun * Omega_sun * cos(theta_sun);
+  * single scattering (max_depth 1, black ground): FP64 double quadrature of
+    the single-scattering integral along the sky-view LUT pixel centres;
+  * reference-form vs carried optical depth, isotropic vs lobe sampling,
+    delta vs ratio shadow tracking: the same expectation (z-tests);
+  * float vs double oracle: the same paths.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy.integrate import quad
+
+import oracle as O
+
+VACUUM = dict(rayleigh_sca_550_m_inv=0.0, ozone_peak_m_inv=0.0, aerosol_sca_550_m_inv=0.0,
+              aerosol_abs_550_m_inv=0.0)
+
+
+def _obs(h=0.2):
+    return np.array([0.0, 6360.0 + h, 0.0])
+
+
+@pytest.mark.parametrize("ratio", [False, True])
+def test_transmittance_is_exp_minus_tau(ratio):
+    p = O.Params(lambda_nm=[420.0, 680.0])
+    for b in range(2):
+        for d in ([0, 1, 0], [1, 0.02, 0], [0.8, 0.6, 0], [1, -0.002, 0]):
+            d = np.array(d, dtype=float) / np.linalg.norm(d)
+            hit, tmax, kind = O.next_boundary_reference(p, _obs(), d)
+            T = math.exp(-O.optical_depth(p, b, _obs(), d, tmax))
+            n = 200000
+            est = O.transmittance_mc(p, b, _obs(), d, n, ratio)
+            sd = math.sqrt(T * (1 - T) / n) if not ratio else 0.5 * math.sqrt(T * (1 - T) / n)
+            assert abs(est - T) <= 4 * sd + 1e-6, (b, d, est, T)
+
+
+def test_vacuum_sun_pixel():
+    """Sun centred in pixel (64, 28) of a 128x64 panorama."""
+    el, az = 9.84375, 1.40625
+    p = O.Params(width=128, height=64, spp=65536, sun_elevation_deg=el, sun_azimuth_deg=az,
+                 ground_albedo=0.0, lambda_nm=list(O.midpoint_grid(16)), **VACUUM)
+    film, m2, _, rc = O.render_pixels(p, [28 * 128 + 64], "f32", threads=1)
+    assert rc == 0
+    lsun = O.lambda_table(p)[:, 4]
+    r = p.sun_angular_radius_rad
+    omega = 2 * math.pi * (1 - math.cos(r))
+    p_hit = omega / math.cos(math.radians(el)) / ((2 * math.pi / 128) * (math.pi / 64))
+    frac = film[0] / lsun
+    n = p.spp * 16
+    est = frac.mean()
+    assert abs(est - p_hit) < 4 * math.sqrt(p_hit * (1 - p_hit) / n)
+    # and no other pixel sees anything
+    film2, _, _, _ = O.render_pixels(p, [28 * 128 + 66, 10 * 128 + 64], "f32", threads=1)
+    assert np.all(film2 == 0.0)
+
+
+def test_bare_ground_lambertian():
+    el = 10.0
+    p = O.Params(width=64, height=32, spp=4096, sun_elevation_deg=el, ground_albedo=0.3,
+                 lambda_nm=[450.0, 600.0], **VACUUM)
+    film, m2, _, rc = O.render_pixels(p, [31 * 64 + 10, 31 * 64 + 40], "f64", threads=2)
+    assert rc == 0
+    lsun = O.lambda_table(p)[:, 4]
+    omega = 2 * math.pi * (1 - math.cos(p.sun_angular_radius_rad))
+    ref = np.tile(0.3 / math.pi * lsun * omega * math.sin(math.radians(el)), (2, 1))
+    # the sun disk spans +-2.6 % in cos(theta) at 10 deg: that is the only noise
+    sd = np.sqrt(np.maximum(m2 - film ** 2, 0) / p.spp)
+    assert np.all(sd < 2e-3 * ref)
+    assert np.all(np.abs(film - ref) < 4 * sd + 1e-5 * ref)
+
+
+def _single_scatter(p, band, d):
+    lam = O.lambda_table(p)[band]
+    sun = O.sun_direction(p)
+    o = _obs(p.observer_altitude_km)
+    hit, tmax, kind = O.next_boundary_reference(p, o, d)
+    mu = float(np.dot(d, sun))
+    pr = O.phase_rayleigh(mu)
+    ph = O.phase_hg(p.aerosol_g, mu)
+    omega = 2 * math.pi * (1 - math.cos(p.sun_angular_radius_rad))
+    R = p.ground_radius_km
+
+    def integrand(t):
+        x = o + t * d
+        h = np.linalg.norm(x) - R
+        if h < 0 or h > 100:
+            return 0.0
+        sr, sas, so, saa = O.coefficients(p, band, h)
+        hs, ts, ks = O.next_boundary_reference(p, x + 0.005 * sun, sun)
+        if ks != 1:
+            return 0.0
+        tau_view = O.optical_depth(p, band, o, d, t)
+        tau_sun = O.optical_depth(p, band, x + 0.005 * sun, sun, ts)
+        return math.exp(-tau_view - tau_sun) * (sr * pr + sas * ph) * lam[4] * omega
+
+    brk = [t for t in np.linspace(0, tmax, 40)[1:-1]]
+    val, err = quad(integrand, 0.0, tmax, points=brk, limit=400, epsrel=1e-7)
+    return val
+
+
+@pytest.mark.slow
+def test_single_scattering_matches_quadrature():
+    p = O.Params(width=8, height=8, spp=40000, projection=O.PROJ_SKY_VIEW_LUT, max_depth=1,
+                 ground_albedo=0.0, sun_elevation_deg=10.0, lambda_nm=[450.0],
+                 flags=O.FLAG_CARRIED_TAU)
+    pixels = [1 * 8 + 1, 2 * 8 + 6, 4 * 8 + 3, 5 * 8 + 7]
+    film, m2, _, rc = O.render_pixels(p, pixels, "f64", threads=4)
+    assert rc == 0
+    for i, pix in enumerate(pixels):
+        x, y = pix % 8, pix // 8
+        d = O.camera_dir(p, (x + 0.5) / 8, (y + 0.5) / 8)
+        ref = _single_scatter(p, 0, d)
+        sd = math.sqrt(max(m2[i, 0] - film[i, 0] ** 2, 0) / p.spp)
+        assert abs(film[i, 0] - ref) < 4 * sd + 1e-9 * ref, (pix, film[i, 0], ref, sd)
+
+
+def _zsum(flags_a, flags_b, seed_b=2, **kw):
+    base = dict(width=64, height=32, spp=32, sun_elevation_deg=10.0, lambda_nm=list(O.midpoint_grid(8)))
+    base.update(kw)
+    fa, ma, _, _, ra = O.render(O.Params(flags=flags_a, **base), "f32", want_m2=True)
+    fb, mb, _, _, rb = O.render(O.Params(flags=flags_b, seed=seed_b, **base), "f32", want_m2=True)
+    assert ra == 0 and rb == 0
+    n = base["spp"]
+    va = np.maximum(ma - fa ** 2, 0) / n
+    vb = np.maximum(mb - fb ** 2, 0) / n
+    # exclude the sun's own pixels, whose binary hit dominates the variance
+    mask = fa.max(axis=2, keepdims=True) < 50
+    mask = np.broadcast_to(mask, fa.shape)
+    z_tot = (fa[mask].sum() - fb[mask].sum()) / math.sqrt(va[mask].sum() + vb[mask].sum())
+    z = (fa - fb)[mask] / np.sqrt(va[mask] + vb[mask] + 1e-300)
+    return z_tot, z
+
+
+@pytest.mark.parametrize("flags_b", [O.FLAG_CARRIED_TAU, O.FLAG_PHASE_IS, O.FLAG_RATIO_TRACKING,
+                                     O.FLAG_CARRIED_TAU | O.FLAG_PHASE_IS | O.FLAG_RATIO_TRACKING])
+def test_estimator_forms_agree(flags_b):
+    z_tot, z = _zsum(0, flags_b)
+    assert abs(z_tot) < 4.0
+    assert abs(np.mean(z)) < 0.05 and 0.9 < np.std(z) < 1.1
+
+
+def test_majorant_layer_count_same_paths():
+    """Delta tracking samples the exact free-flight law for any majorant, so
+    16 and 64 shells give equal expectations (independent streams here)."""
+    base = dict(width=64, height=32, spp=32, sun_elevation_deg=10.0, lambda_nm=list(O.midpoint_grid(8)),
+                flags=O.FLAG_CARRIED_TAU)
+    fa, ma, _, _, _ = O.render(O.Params(majorant_layers=64, **base), "f32", want_m2=True)
+    fb, mb, _, _, _ = O.render(O.Params(majorant_layers=16, seed=3, **base), "f32", want_m2=True)
+    mask = np.broadcast_to(fa.max(axis=2, keepdims=True) < 50, fa.shape)
+    va = np.maximum(ma - fa ** 2, 0) / 32
+    vb = np.maximum(mb - fb ** 2, 0) / 32
+    z_tot = (fa[mask].sum() - fb[mask].sum()) / math.sqrt(va[mask].sum() + vb[mask].sum())
+    assert abs(z_tot) < 4.0
+
+
+def test_float_and_double_oracles_pair():
+    p = O.Params(width=32, height=16, spp=1, flags=O.FLAG_CARRIED_TAU | O.FLAG_PHASE_IS)
+    f32, _, _, _, _ = O.render(p, "f32")
+    f64, _, _, _, _ = O.render(p, "f64")
+    scale = f64.max()
+    close = np.abs(f32 - f64) <= np.maximum(1e-3 * np.abs(f64), 1e-6 * scale)
+    assert close.mean() > 0.99
+
+
+def test_oracle_watchdog_and_invalid():
+    p = O.Params(width=8, height=4, spp=1, max_depth=0, watchdog_limit=0)
+    _, _, _, _, rc = O.render(p, "f32")
+    assert rc == 6
+    p = O.Params(width=8, height=4, spp=1, max_segments=0)
+    _, _, _, _, rc = O.render(p, "f32")
+    assert rc == 6
+    p = O.Params(width=0)
+    _, _, _, _, rc = O.render(p, "f32")
+    assert rc == 1
