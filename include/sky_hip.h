@@ -119,6 +119,7 @@ typedef struct sky_tuning {
     uint32_t pixels_per_item; /* pixels per wave work item (128 / count)       */
     uint32_t event_threshold; /* lanes waiting before events are handled (32)  */
     uint32_t track_steps;     /* tracking micro-steps between handler passes (8) */
+    uint32_t occupancy;       /* 0: compiler's register budget; 4: 128-VGPR variant */
 } sky_tuning;
 
 /* ---- context ---- */
@@ -162,6 +163,36 @@ sky_status sky_render(sky_context *ctx, const sky_planet *planet, const sky_atmo
 sky_status sky_render_device(sky_context *ctx, const sky_planet *planet, const sky_atmosphere *atm,
                              const sky_wavelengths *wl, const sky_render_config *cfg, float *d_film,
                              float *d_film_m2, float *d_xyz, void *stream, sky_render_stats *stats);
+
+/* ---- ragged batch: many renders in one persistent launch ----
+ * The reference renders one image per process and device
+ * (scripts/bake_skyview_elevations.ps1:13-35 calls the CLI per elevation);
+ * this runs a whole batch of independent renders -- each with its own
+ * scene, resolution, wavelength set and spp -- as one wave-pool launch,
+ * captured in a HIP graph and replayed when the batch layout repeats.
+ * Outputs are host buffers per item (either may be NULL). */
+typedef struct sky_batch_item {
+    const sky_planet *planet;
+    const sky_atmosphere *atm;
+    const sky_wavelengths *wl;
+    const sky_render_config *cfg;
+    float *xyz;  /* [H*W*3]     */
+    float *film; /* [H*W*count] */
+} sky_batch_item;
+
+typedef struct sky_batch_stats {
+    double kernel_ms;      /* device time of the launch                      */
+    double total_ms;       /* host wall time of the call                     */
+    uint64_t path_samples; /* sum over renders of W * H * spp * count        */
+    uint32_t renders, items, watchdog, graph_reused;
+    double latency_p50_ms, latency_p99_ms, latency_max_ms; /* completion after launch */
+} sky_batch_stats;
+
+#define SKY_BATCH_NO_GRAPH 1u /* launch eagerly instead of through the cached graph */
+
+/* finish_ms (may be NULL): per item, completion time after the launch began. */
+sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint32_t n, uint32_t flags,
+                            double *finish_ms, sky_batch_stats *stats);
 
 /* ---- deterministic optical depth (Gauss-Legendre per shell) ----
  * rays: n x 7 doubles (origin xyz km, direction xyz, t_max km);

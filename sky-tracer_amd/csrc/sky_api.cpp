@@ -8,6 +8,7 @@
 // XYZ happen on the device, and only the requested outputs are copied back.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -52,10 +53,17 @@ struct DevBuf {
 struct sky_context {
     int device = 0;
     int num_cus = 256;
+    double clock_khz = 100000.0;  // wall_clock64 rate
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     sky_tuning tuning{};
-    DevBuf lam, maj, film, film_m2, xyz, accum, counter, diag, rays, tau, lamd;
+    DevBuf lam, maj, film, film_m2, xyz, accum, meta, diag, rays, tau, lamd, jobs;
+    std::vector<sky::DevJob> host_jobs;  // staging, alive until the stream syncs
+    // ragged batches
+    DevBuf b_lam, b_maj, b_xyz, b_film, b_accum, b_meta, b_jobs;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    uint64_t graph_key = 0;
 };
 
 namespace {
@@ -80,25 +88,37 @@ sky_status run_render(sky_context *ctx, sky::HostScene &hs, hipStream_t stream,
     HIP_TRY(ctx->lam.ensure(hs.lam.size() * sizeof(sky::DevLambda)), "alloc lambda table", SKY_ERR_OOM);
     HIP_TRY(ctx->maj.ensure(hs.maj.size() * sizeof(float)), "alloc majorants", SKY_ERR_OOM);
     HIP_TRY(ctx->diag.ensure(64), "alloc diagnostics", SKY_ERR_OOM);
-    HIP_TRY(ctx->counter.ensure(64), "alloc work counter", SKY_ERR_OOM);
+    HIP_TRY(ctx->meta.ensure(64), "alloc work counters", SKY_ERR_OOM);
+    HIP_TRY(ctx->jobs.ensure(sizeof(sky::DevJob)), "alloc job table", SKY_ERR_OOM);
     HIP_TRY(hipMemcpyAsync(ctx->lam.p, hs.lam.data(), hs.lam.size() * sizeof(sky::DevLambda),
                            hipMemcpyHostToDevice, stream),
             "upload lambda table", SKY_ERR_COPY);
     HIP_TRY(hipMemcpyAsync(ctx->maj.p, hs.maj.data(), hs.maj.size() * sizeof(float),
                            hipMemcpyHostToDevice, stream),
             "upload majorants", SKY_ERR_COPY);
-    HIP_TRY(hipMemsetAsync(ctx->diag.p, 0, 64, stream), "clear diagnostics", SKY_ERR_SYNC);
+    HIP_TRY(sky::launch_zero((uint32_t *)ctx->diag.p, 16, stream), "clear diagnostics", SKY_ERR_SYNC);
     s.lam = (const sky::DevLambda *)ctx->lam.p;
     s.maj = (const float *)ctx->maj.p;
     s.diag = (uint32_t *)ctx->diag.p;
     const bool simple = (s.flags & SKY_FLAG_SIMPLE_KERNEL) != 0;
     sky::LaunchShape shape;
     sky::WavePlan plan;
+    const uint32_t npx = s.width * s.height;
     if (!simple) {
-        plan = sky::plan_wavepool(s, ctx->tuning, ctx->num_cus);
-        HIP_TRY(ctx->accum.ensure(plan.accum_bytes), "alloc partial sums", SKY_ERR_OOM);
+        ctx->host_jobs.assign(1, sky::DevJob{});
+        sky::DevJob &j = ctx->host_jobs[0];
+        j.s = s;
+        sky::plan_job(s, ctx->tuning, ctx->num_cus, sky::launch_min_items(ctx->tuning, ctx->num_cus), j);
+        j.item_base = 0;
+        HIP_TRY(ctx->accum.ensure(sky::job_accum_bytes(j)), "alloc partial sums", SKY_ERR_OOM);
+        j.accum = (float2 *)ctx->accum.p;
+        j.done = (uint32_t *)ctx->meta.p + 4;
+        j.t_done = (unsigned long long *)((uint32_t *)ctx->meta.p + 6);
+        HIP_TRY(hipMemcpyAsync(ctx->jobs.p, &j, sizeof(j), hipMemcpyHostToDevice, stream), "upload job",
+                SKY_ERR_COPY);
+        plan = sky::plan_launch(ctx->tuning, ctx->num_cus, j.n_items);
     } else if (!s.film) {
-        size_t bytes = (size_t)s.width * s.height * s.nlam * sizeof(float);
+        size_t bytes = (size_t)npx * s.nlam * sizeof(float);
         HIP_TRY(ctx->film.ensure(bytes), "alloc film", SKY_ERR_OOM);
         s.film = (float *)ctx->film.p;
     }
@@ -107,7 +127,7 @@ sky_status run_render(sky_context *ctx, sky::HostScene &hs, hipStream_t stream,
         HIP_TRY(sky::launch_simple(s, stream, shape), "launch simple integrator", SKY_ERR_SYNC);
         if (s.xyz) HIP_TRY(sky::launch_film_to_xyz(s, stream), "launch xyz", SKY_ERR_SYNC);
     } else {
-        HIP_TRY(sky::launch_wavepool(s, plan, ctx->accum.p, (uint32_t *)ctx->counter.p, stream),
+        HIP_TRY(sky::launch_pool((const sky::DevJob *)ctx->jobs.p, 1, npx, plan, (uint32_t *)ctx->meta.p, 8, stream),
                 "launch wave-pool integrator", SKY_ERR_SYNC);
         shape.grid = plan.grid;
         shape.block = plan.block;
@@ -204,9 +224,11 @@ sky_status sky_context_create(int device, sky_context **out)
         delete c;
         return hip_fail(e, "device initialisation", SKY_ERR_DEVICE_INIT);
     }
-    int cus = 0;
+    int cus = 0, khz = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
         c->num_cus = cus;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+        c->clock_khz = khz;
     *out = c;
     return SKY_OK;
 }
@@ -216,8 +238,11 @@ void sky_context_destroy(sky_context *c)
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (DevBuf *b : {&c->lam, &c->maj, &c->film, &c->film_m2, &c->xyz, &c->accum, &c->counter, &c->diag,
-                      &c->rays, &c->tau, &c->lamd})
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    for (DevBuf *b : {&c->lam, &c->maj, &c->film, &c->film_m2, &c->xyz, &c->accum, &c->meta, &c->diag,
+                      &c->rays, &c->tau, &c->lamd, &c->jobs, &c->b_lam, &c->b_maj, &c->b_xyz, &c->b_film,
+                      &c->b_accum, &c->b_meta, &c->b_jobs})
         b->release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -228,7 +253,8 @@ void sky_context_destroy(sky_context *c)
 sky_status sky_context_set_tuning(sky_context *c, const sky_tuning *t)
 {
     if (!c || !t) return fail(SKY_ERR_INVALID_CONFIG, "null context or tuning");
-    if (t->event_threshold > 64 || t->pixels_per_item > 256 || t->waves_per_cu > 64 || t->track_steps > 4096)
+    if (t->event_threshold > 64 || t->pixels_per_item > 256 || t->waves_per_cu > 64 || t->track_steps > 4096 ||
+        (t->occupancy != 0 && t->occupancy != 4))
         return fail(SKY_ERR_INVALID_CONFIG, "tuning value out of range");
     c->tuning = *t;
     return SKY_OK;
@@ -339,6 +365,191 @@ sky_status sky_render(sky_context *ctx, const sky_planet *planet, const sky_atmo
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (st_copy != SKY_OK) return st_copy;
     if (st == SKY_ERR_WATCHDOG) return fail(SKY_ERR_WATCHDOG, "GPU path watchdog was reached");
+    return SKY_OK;
+}
+
+sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint32_t n, uint32_t flags,
+                            double *finish_ms, sky_batch_stats *stats)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    if (!items || n == 0) return fail(SKY_ERR_INVALID_CONFIG, "empty batch");
+    std::vector<sky::HostScene> scenes(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        std::string err;
+        sky_status st = sky::build_scene(items[i].planet, items[i].atm, items[i].wl, items[i].cfg, scenes[i], err);
+        if (st != SKY_OK) return fail(st, "batch item " + std::to_string(i) + ": " + err);
+        if (scenes[i].dev.flags & SKY_FLAG_SIMPLE_KERNEL)
+            return fail(SKY_ERR_INVALID_CONFIG, "batch item " + std::to_string(i) + ": batches use the wave-pool kernel");
+    }
+    if (!ctx) return fail(SKY_ERR_NO_DEVICE, "no device context");
+    HIP_TRY(hipSetDevice(ctx->device), "set device", SKY_ERR_DEVICE_INIT);
+    hipStream_t stream = ctx->stream;
+
+    // Largest renders first (longest-processing-time order), so the launch
+    // ends on small items.
+    std::vector<uint32_t> order(n);
+    for (uint32_t i = 0; i < n; ++i) order[i] = i;
+    auto cost = [&](uint32_t i) {
+        const sky::DevScene &d = scenes[i].dev;
+        return (double)d.width * d.height * d.nlam * d.spp;
+    };
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost(a) > cost(b); });
+
+    // Table offsets.
+    size_t n_lam = 0, n_maj = 0, n_xyz = 0, n_film = 0, n_acc = 0;
+    uint32_t max_npx = 0, total_items = 0;
+    std::vector<size_t> o_lam(n), o_maj(n), o_xyz(n), o_film(n), o_acc(n);
+    ctx->host_jobs.assign(n, sky::DevJob{});
+    const uint32_t min_items = 0;  // the batch balances across renders; no sample split needed
+    uint64_t samples = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        uint32_t i = order[k];
+        sky::HostScene &h = scenes[i];
+        uint32_t npx = h.dev.width * h.dev.height;
+        o_lam[i] = n_lam;
+        o_maj[i] = n_maj;
+        o_xyz[i] = n_xyz;
+        o_film[i] = n_film;
+        n_lam += h.lam.size();
+        n_maj += h.maj.size();
+        n_xyz += (size_t)npx * 3;
+        if (items[i].film) n_film += (size_t)npx * h.nlam;
+        sky::DevJob &j = ctx->host_jobs[k];
+        j.s = h.dev;
+        sky::plan_job(h.dev, ctx->tuning, ctx->num_cus, min_items, j);
+        j.item_base = total_items;
+        total_items += j.n_items;
+        o_acc[i] = n_acc;
+        n_acc += sky::job_accum_bytes(j) / sizeof(float2);
+        max_npx = npx > max_npx ? npx : max_npx;
+        samples += (uint64_t)npx * h.dev.spp * h.dev.nlam;
+    }
+    const uint32_t meta_words = 8 + 4 * n;
+    HIP_TRY(ctx->b_lam.ensure(n_lam * sizeof(sky::DevLambda)), "alloc batch lambda tables", SKY_ERR_OOM);
+    HIP_TRY(ctx->b_maj.ensure(n_maj * sizeof(float)), "alloc batch majorants", SKY_ERR_OOM);
+    HIP_TRY(ctx->b_xyz.ensure(n_xyz * sizeof(float)), "alloc batch xyz", SKY_ERR_OOM);
+    if (n_film) HIP_TRY(ctx->b_film.ensure(n_film * sizeof(float)), "alloc batch film", SKY_ERR_OOM);
+    HIP_TRY(ctx->b_accum.ensure(n_acc * sizeof(float2)), "alloc batch partial sums", SKY_ERR_OOM);
+    HIP_TRY(ctx->b_meta.ensure(meta_words * sizeof(uint32_t)), "alloc batch counters", SKY_ERR_OOM);
+    HIP_TRY(ctx->b_jobs.ensure(n * sizeof(sky::DevJob)), "alloc batch job table", SKY_ERR_OOM);
+    HIP_TRY(ctx->diag.ensure(64), "alloc diagnostics", SKY_ERR_OOM);
+    std::vector<sky::DevLambda> all_lam(n_lam);
+    std::vector<float> all_maj(n_maj);
+    uint32_t *meta = (uint32_t *)ctx->b_meta.p;
+    for (uint32_t k = 0; k < n; ++k) {
+        uint32_t i = order[k];
+        sky::HostScene &h = scenes[i];
+        std::copy(h.lam.begin(), h.lam.end(), all_lam.begin() + o_lam[i]);
+        std::copy(h.maj.begin(), h.maj.end(), all_maj.begin() + o_maj[i]);
+        sky::DevJob &j = ctx->host_jobs[k];
+        j.s.lam = (const sky::DevLambda *)ctx->b_lam.p + o_lam[i];
+        j.s.maj = (const float *)ctx->b_maj.p + o_maj[i];
+        j.s.xyz = (float *)ctx->b_xyz.p + o_xyz[i];
+        j.s.film = items[i].film ? (float *)ctx->b_film.p + o_film[i] : nullptr;
+        j.s.film_m2 = nullptr;
+        j.s.diag = (uint32_t *)ctx->diag.p;
+        j.accum = (float2 *)ctx->b_accum.p + o_acc[i];
+        j.done = meta + 8 + 2 * k;
+        j.t_done = (unsigned long long *)(meta + 8 + 2 * n + 2 * k);
+    }
+    HIP_TRY(hipMemcpyAsync(ctx->b_lam.p, all_lam.data(), n_lam * sizeof(sky::DevLambda), hipMemcpyHostToDevice,
+                           stream), "upload batch lambda tables", SKY_ERR_COPY);
+    HIP_TRY(hipMemcpyAsync(ctx->b_maj.p, all_maj.data(), n_maj * sizeof(float), hipMemcpyHostToDevice, stream),
+            "upload batch majorants", SKY_ERR_COPY);
+    HIP_TRY(hipMemcpyAsync(ctx->b_jobs.p, ctx->host_jobs.data(), n * sizeof(sky::DevJob), hipMemcpyHostToDevice,
+                           stream), "upload batch job table", SKY_ERR_COPY);
+    HIP_TRY(sky::launch_zero((uint32_t *)ctx->diag.p, 16, stream), "clear diagnostics", SKY_ERR_SYNC);
+    sky::WavePlan plan = sky::plan_launch(ctx->tuning, ctx->num_cus, total_items);
+
+    // The graph holds only kernel nodes (zero, integrate, finalize); the
+    // tables it reads are uploaded above on every call.
+    const bool use_graph = !(flags & SKY_BATCH_NO_GRAPH);
+    uint64_t key = 1469598103934665603ull;
+    auto mix = [&](uint64_t v) { key = (key ^ v) * 1099511628211ull; };
+    mix(n);
+    mix(total_items);
+    mix(max_npx);
+    mix((uint64_t)(uintptr_t)ctx->b_jobs.p);
+    mix((uint64_t)(uintptr_t)ctx->b_meta.p);
+    mix(plan.grid);
+    mix(plan.event_threshold);
+    mix(plan.track_steps);
+    mix(plan.occupancy);
+    bool reused = false;
+    HIP_TRY(hipEventRecord(ctx->ev0, stream), "event record", SKY_ERR_SYNC);
+    if (use_graph) {
+        if (!ctx->graph_exec || ctx->graph_key != key) {
+            if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+            if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+            ctx->graph_exec = nullptr;
+            ctx->graph = nullptr;
+            HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture", SKY_ERR_SYNC);
+            hipError_t le = sky::launch_pool((const sky::DevJob *)ctx->b_jobs.p, n, max_npx, plan, meta,
+                                             meta_words, stream);
+            hipError_t ce = hipStreamEndCapture(stream, &ctx->graph);
+            if (le != hipSuccess) return hip_fail(le, "capture batch launch", SKY_ERR_SYNC);
+            if (ce != hipSuccess) return hip_fail(ce, "end capture", SKY_ERR_SYNC);
+            HIP_TRY(hipGraphInstantiate(&ctx->graph_exec, ctx->graph, nullptr, nullptr, 0), "instantiate graph",
+                    SKY_ERR_SYNC);
+            ctx->graph_key = key;
+        } else {
+            reused = true;
+        }
+        HIP_TRY(hipGraphLaunch(ctx->graph_exec, stream), "launch graph", SKY_ERR_SYNC);
+    } else {
+        HIP_TRY(sky::launch_pool((const sky::DevJob *)ctx->b_jobs.p, n, max_npx, plan, meta, meta_words, stream),
+                "launch batch", SKY_ERR_SYNC);
+    }
+    HIP_TRY(hipEventRecord(ctx->ev1, stream), "event record", SKY_ERR_SYNC);
+    std::vector<uint32_t> host_meta(meta_words);
+    uint32_t wd = 0;
+    HIP_TRY(hipMemcpyAsync(host_meta.data(), meta, meta_words * sizeof(uint32_t), hipMemcpyDeviceToHost, stream),
+            "read batch counters", SKY_ERR_COPY);
+    HIP_TRY(hipMemcpyAsync(&wd, ctx->diag.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream), "read diagnostics",
+            SKY_ERR_COPY);
+    for (uint32_t i = 0; i < n; ++i) {
+        const sky::HostScene &h = scenes[i];
+        size_t npx = (size_t)h.dev.width * h.dev.height;
+        if (items[i].xyz)
+            HIP_TRY(hipMemcpyAsync(items[i].xyz, (float *)ctx->b_xyz.p + o_xyz[i], npx * 3 * sizeof(float),
+                                   hipMemcpyDeviceToHost, stream), "copy batch xyz", SKY_ERR_COPY);
+        if (items[i].film)
+            HIP_TRY(hipMemcpyAsync(items[i].film, (float *)ctx->b_film.p + o_film[i], npx * h.nlam * sizeof(float),
+                                   hipMemcpyDeviceToHost, stream), "copy batch film", SKY_ERR_COPY);
+    }
+    HIP_TRY(hipStreamSynchronize(stream), "synchronize", SKY_ERR_SYNC);
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    // per-render completion times from the device clock
+    unsigned long long t_start = (unsigned long long)host_meta[2] | ((unsigned long long)host_meta[3] << 32);
+    std::vector<double> lat(n);
+    for (uint32_t k = 0; k < n; ++k) {
+        uint32_t i = order[k];
+        size_t w = 8 + 2 * n + 2 * k;
+        unsigned long long td = (unsigned long long)host_meta[w] | ((unsigned long long)host_meta[w + 1] << 32);
+        lat[i] = td >= t_start ? (double)(td - t_start) / ctx->clock_khz : 0.0;
+        if (finish_ms) finish_ms[i] = lat[i];
+    }
+    if (stats) {
+        std::vector<double> sorted = lat;
+        std::sort(sorted.begin(), sorted.end());
+        auto pct = [&](double q) {
+            size_t idx = (size_t)std::ceil(q * sorted.size()) - 1;
+            return sorted[idx < sorted.size() ? idx : sorted.size() - 1];
+        };
+        stats->kernel_ms = ms;
+        stats->path_samples = samples;
+        stats->renders = n;
+        stats->items = total_items;
+        stats->watchdog = wd;
+        stats->graph_reused = reused;
+        stats->latency_p50_ms = pct(0.50);
+        stats->latency_p99_ms = pct(0.99);
+        stats->latency_max_ms = sorted.back();
+        stats->total_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    if (wd) return fail(SKY_ERR_WATCHDOG, "GPU path watchdog was reached");
     return SKY_OK;
 }
 

@@ -41,16 +41,36 @@ struct LaunchShape {
 // Kernel launchers (sky_kernels.hip).  Device pointers in `s` must be set.
 hipError_t launch_simple(const DevScene &s, hipStream_t stream, LaunchShape &shape);
 hipError_t launch_film_to_xyz(const DevScene &s, hipStream_t stream);
-// Work split of the wave-pool integrator (sky_wavepool.hip).
-struct WavePlan {
-    uint32_t pixels_per_item = 1, groups = 0, chunks = 1, samples_per_chunk = 0, n_items = 0;
-    uint32_t grid = 0, block = 256, event_threshold = 32, track_steps = 8;
-    size_t accum_bytes = 0;
+// One render of a wave-pool launch: its scene, its work split and where its
+// partial sums and completion record go (device memory).
+struct DevJob {
+    DevScene s;
+    uint32_t pixels_per_item, groups, chunks, samples_per_chunk, n_items, item_base;
+    float2 *accum;              // [chunk][pixel][band] (sum, sum of squares)
+    uint32_t *done;             // items finished
+    unsigned long long *t_done; // device clock when the last item finished
 };
-WavePlan plan_wavepool(const DevScene &s, const sky_tuning &t, int num_cus);
-// accum: plan.accum_bytes of device memory; counter: one device uint32.
-hipError_t launch_wavepool(const DevScene &s, const WavePlan &plan, void *accum,
-                           uint32_t *counter, hipStream_t stream);
+
+struct PoolArgs {
+    const DevJob *jobs;
+    uint32_t n_jobs, total_items;
+    uint32_t *counter;
+    unsigned long long *t_start;
+    uint32_t event_threshold, track_steps;
+};
+
+// Launch-wide settings of the wave-pool integrator (sky_wavepool.hip).
+struct WavePlan {
+    uint32_t n_items = 0, grid = 0, block = 256, event_threshold = 32, track_steps = 8, occupancy = 0;
+};
+
+void plan_job(const DevScene &s, const sky_tuning &t, int num_cus, uint32_t min_items, DevJob &j);
+size_t job_accum_bytes(const DevJob &j);
+uint32_t launch_min_items(const sky_tuning &t, int num_cus);
+WavePlan plan_launch(const sky_tuning &t, int num_cus, uint32_t total_items);
+hipError_t launch_pool(const DevJob *d_jobs, uint32_t n_jobs, uint32_t max_npx, const WavePlan &p,
+                       uint32_t *meta, uint32_t meta_words, hipStream_t stream);
+hipError_t launch_zero(uint32_t *p, uint32_t n, hipStream_t stream);
 hipError_t launch_optical_depth(const DevScene &s, const double *rays, uint32_t n,
                                 const double *dev_lambda_d, double *tau, hipStream_t stream);
 

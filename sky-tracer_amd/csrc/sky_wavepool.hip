@@ -33,18 +33,6 @@ constexpr int kSlotsPerWave = 256;
 enum LaneState : int { ST_NEED = 0, ST_EVENT = 1, ST_TRACK = 2, ST_IDLE = 3 };
 enum LaneEvent : int { EV_START = 0, EV_HIT = 1, EV_BOUNDARY = 2, EV_SHADOW = 3 };
 
-struct WorkDesc {
-    uint32_t pixels_per_item;  // P, P * nlam <= kSlotsPerWave
-    uint32_t groups;           // ceil(npx / P)
-    uint32_t chunks;           // sample chunks C
-    uint32_t samples_per_chunk;
-    uint32_t n_items;          // groups * chunks
-    uint32_t event_threshold;
-    uint32_t track_steps;
-    float2 *accum;             // [chunk][pixel][band] (sum, sum of squares)
-    uint32_t *counter;
-};
-
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 
 __device__ __forceinline__ uint32_t mbcnt(uint64_t m)
@@ -54,50 +42,82 @@ __device__ __forceinline__ uint32_t mbcnt(uint64_t m)
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
-// Everything a lane keeps between loop trips.
+// Everything a lane keeps between loop trips.  The main ray starts at the
+// current vertex (px, py, pz): the camera position, or the last vertex moved
+// by the ray epsilon, exactly as the reference offsets new rays
+// (sky_trace.wgsl:744, 766).  During a shadow ray the vertex stays put.
 struct LaneCtx {
-    int st, ev;
+    int st, ev, mode, vtype, kind, pend;
     uint32_t slot, band;
-    DevLambda l;
+    float br, bo, bas, baa;  // this band's coefficients (medium evaluation)
     Rng rng;
     float tp, L;
     uint32_t depth;
-    // current vertex
-    double px, py, pz;
-    f3 vin;
-    int vtype;  // 0 scattering, 1 ground
-    float nee_sum, nee_factor;
-    float rw;  // sigma_R / sigma_s at a scattering vertex (phase mixture weight)
+    double px, py, pz;  // current vertex / main-ray origin (FP64)
+    f3 vin;             // incoming direction at the vertex (camera direction at start)
+    float nee_sum, nee_factor, rw;
     uint32_t nee_i;
-    // pending main ray (origin, direction) between vertices
-    f3 dir;
     // ray being tracked
-    Ray ray;
-    int mode;  // TrackMode
-    float t, tau, seg_end, maj, T, ratio;
-    int k, out, kn, on, pend;
+    f3 d;
+    float bf, qf, tmax;
+    double q;
+    float t, tau, seg_end, maj;
+    float aux;  // accept ratio pending (delta) / transmittance weight (ratio, shadow result)
+    int k, out, kn, on;
     uint32_t guard;
 };
 
 __device__ __forceinline__ void seg_begin(const DevScene &s, LaneCtx &c)
 {
-    float tc = next_crossing(s, c.ray, c.k, c.out, c.kn, c.on);
-    c.seg_end = fmaxf(fminf(tc, c.ray.tmax), c.t);
+    Ray r;
+    r.q = c.q;
+    r.bf = c.bf;
+    float tc = next_crossing(s, r, c.k, c.out, c.kn, c.on);
+    c.seg_end = fmaxf(fminf(tc, c.tmax), c.t);
     c.maj = s.maj[c.band * s.L + c.k];
 }
 
-__device__ __forceinline__ void track_begin(const DevScene &s, LaneCtx &c, int mode)
+// Sets up a ray from (ox, oy, oz) along d (FP64 set-up, sky_device.hpp) and
+// starts tracking it; returns false if it has no boundary.
+__device__ __forceinline__ bool ray_begin(const DevScene &s, LaneCtx &c, double ox, double oy,
+                                          double oz, f3 d, int mode)
 {
+    Ray r;
+    ray_setup(s, ox, oy, oz, d, r);
+    c.d = d;
+    c.bf = r.bf;
+    c.qf = r.qf;
+    c.q = r.q;
+    c.tmax = r.tmax;
+    c.kind = r.kind;
     c.mode = mode;
     c.t = 0.0f;
     c.tau = -1.0f;
-    c.T = 1.0f;
+    c.aux = 1.0f;
     c.pend = 0;
     c.guard = 1;
-    c.k = c.ray.layer;
-    c.out = c.ray.outward;
+    c.k = r.layer;
+    c.out = r.outward;
     seg_begin(s, c);
     c.st = ST_TRACK;
+    return r.hit != 0;
+}
+
+__device__ __forceinline__ float lane_altitude(const DevScene &s, const LaneCtx &c, float t)
+{
+    float sv = c.bf + t;
+    float w = c.qf + sv * sv;
+    return w / (sqrtf(s.R2f + w) + s.Rf);
+}
+
+__device__ __forceinline__ Coeffs lane_medium(const DevScene &s, const LaneCtx &c, float h)
+{
+    DevLambda l;
+    l.br = c.br;
+    l.bo = c.bo;
+    l.bas = c.bas;
+    l.baa = c.baa;
+    return medium(s, l, h);
 }
 
 // One tracking micro-step: at most one RNG draw.  An empty segment (its
@@ -112,14 +132,16 @@ __device__ __forceinline__ void track_step(const DevScene &s, LaneCtx &c, bool &
     if (need_draw) u = rng_next(c.rng);
     if (c.pend) {
         c.pend = 0;
-        if (u < c.ratio) {
+        if (u < c.aux) {
             c.st = ST_EVENT;
             if (c.mode == kTrackCollision) {
                 c.ev = EV_HIT;
             } else {
-                c.T = 0.0f;
+                c.aux = 0.0f;
                 c.ev = EV_SHADOW;
             }
+        } else {
+            c.aux = 1.0f;
         }
         return;
     }
@@ -146,7 +168,7 @@ __device__ __forceinline__ void track_step(const DevScene &s, LaneCtx &c, bool &
     }
     if (cross) {
         c.t = c.seg_end;
-        if (c.t >= c.ray.tmax) {
+        if (c.t >= c.tmax) {
             c.st = ST_EVENT;
             c.ev = c.mode == kTrackCollision ? EV_BOUNDARY : EV_SHADOW;
             return;
@@ -154,9 +176,9 @@ __device__ __forceinline__ void track_step(const DevScene &s, LaneCtx &c, bool &
         if (c.guard > s.max_segments) {
             wd = true;
             c.st = ST_EVENT;
-            c.T = 0.0f;
+            c.aux = 0.0f;
             c.ev = c.mode == kTrackCollision ? EV_BOUNDARY : EV_SHADOW;
-            c.ray.kind = 2;  // watchdog: ends the path without contribution
+            c.kind = 2;  // watchdog: ends the path without contribution
             return;
         }
         c.guard++;
@@ -166,24 +188,25 @@ __device__ __forceinline__ void track_step(const DevScene &s, LaneCtx &c, bool &
         return;
     }
     c.t = tn;
-    Coeffs co = medium(s, c.l, altitude(s, c.ray, c.t));
+    Coeffs co = lane_medium(s, c, lane_altitude(s, c, c.t));
     float st = co.sr + co.sas + co.so + co.saa;
     float ratio = fminf(fmaxf(st / c.maj, 0.0f), 1.0f);
     if (c.mode == kTrackRatio) {
-        c.T *= 1.0f - ratio;
+        c.aux *= 1.0f - ratio;
     } else {
         c.pend = 1;
-        c.ratio = ratio;
+        c.aux = ratio;
     }
 }
 
-__device__ __forceinline__ void vertex_pos(const Ray &r, float t, double &px, double &py, double &pz)
+// Vertex at distance t along the main ray from (px, py, pz), FP64.
+__device__ __forceinline__ void advance_vertex(LaneCtx &c, float t)
 {
-    double dx = r.d.x, dy = r.d.y, dz = r.d.z;
+    double dx = c.d.x, dy = c.d.y, dz = c.d.z;
     double inv = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
-    px = r.ox + dx * inv * (double)t;
-    py = r.oy + dy * inv * (double)t;
-    pz = r.oz + dz * inv * (double)t;
+    c.px += dx * inv * (double)t;
+    c.py += dy * inv * (double)t;
+    c.pz += dz * inv * (double)t;
 }
 
 enum Act : int { ACT_NONE = 0, ACT_NEE = 1, ACT_CONT = 2, ACT_MAIN = 3, ACT_FINISH = 4, ACT_TRACKING = 5 };
@@ -194,25 +217,26 @@ __device__ __forceinline__ void handle_event(const DevScene &s, LaneCtx &c, floa
 {
     const uint32_t n_light = s.dls ? s.dls : 1u;
     int act = ACT_NONE;
+    f3 mdir = c.vin;  // direction of the next main ray
     if (c.ev == EV_START) {
         act = ACT_MAIN;
     } else if (c.ev == EV_BOUNDARY) {
-        if (c.ray.kind == 1) {
+        if (c.kind == 1) {
             // exit: the sun disk is seen only by the camera ray (wgsl:745-749)
-            if (c.depth == 0 && in_sun_cone(s, c.ray.d)) c.L = c.L + c.tp * c.l.lsun;
+            if (c.depth == 0 && in_sun_cone(s, c.d)) c.L = c.L + c.tp * s.lam[c.band].lsun;
             act = ACT_FINISH;
-        } else if (c.ray.kind == 2) {
+        } else if (c.kind == 2) {
             act = ACT_FINISH;  // tracking watchdog
         } else {
-            vertex_pos(c.ray, c.ray.tmax, c.px, c.py, c.pz);
+            advance_vertex(c, c.tmax);
             c.vtype = 1;
             c.nee_i = 0;
             c.nee_sum = 0.0f;
             act = ACT_NEE;
         }
     } else if (c.ev == EV_HIT) {
-        vertex_pos(c.ray, c.t, c.px, c.py, c.pz);
-        Coeffs co = medium(s, c.l, altitude(s, c.ray, c.t));
+        Coeffs co = lane_medium(s, c, lane_altitude(s, c, c.t));
+        advance_vertex(c, c.t);
         float scat = co.sr + co.sas;
         float ext = scat + co.so + co.saa;
         act = ACT_FINISH;
@@ -220,7 +244,7 @@ __device__ __forceinline__ void handle_event(const DevScene &s, LaneCtx &c, floa
             c.tp = c.tp * fminf(fmaxf(scat / ext, 0.0f), 1.0f);
             if (c.tp > 0.0f) {
                 c.vtype = 0;
-                c.vin = c.ray.d;
+                c.vin = c.d;
                 c.rw = co.sr / scat;
                 c.nee_i = 0;
                 c.nee_sum = 0.0f;
@@ -228,7 +252,7 @@ __device__ __forceinline__ void handle_event(const DevScene &s, LaneCtx &c, floa
             }
         }
     } else {  // EV_SHADOW
-        c.nee_sum += c.nee_factor * c.T;
+        c.nee_sum += c.nee_factor * c.aux;
         c.nee_i++;
         act = c.nee_i < n_light ? ACT_NEE : ACT_CONT;
     }
@@ -239,14 +263,11 @@ __device__ __forceinline__ void handle_event(const DevScene &s, LaneCtx &c, floa
         float a1 = rng_next(c.rng);
         f3 ld = sun_cone_dir(s, a0, a1);
         float factor;
-        double nx = 0, ny = 0, nz = 0;
+        const float lsun = s.lam[c.band].lsun;
         if (c.vtype == 1) {
             double pn = 1.0 / sqrt(c.px * c.px + c.py * c.py + c.pz * c.pz);
-            nx = c.px * pn;
-            ny = c.py * pn;
-            nz = c.pz * pn;
-            float cs = fmaxf((float)nx * ld.x + (float)ny * ld.y + (float)nz * ld.z, 0.0f);
-            factor = s.albedo * kInvPi * c.l.lsun * cs / s.cone_pdf;
+            float cs = fmaxf((float)(c.px * pn) * ld.x + (float)(c.py * pn) * ld.y + (float)(c.pz * pn) * ld.z, 0.0f);
+            factor = s.albedo * kInvPi * lsun * cs / s.cone_pdf;
             if (!(cs > 0.0f)) {
                 c.nee_i++;
                 act = c.nee_i < n_light ? ACT_NEE : ACT_CONT;
@@ -255,17 +276,18 @@ __device__ __forceinline__ void handle_event(const DevScene &s, LaneCtx &c, floa
         } else {
             float mu = dot3(c.vin, ld);
             float ph = c.rw * phase_rayleigh(mu) + (1.0f - c.rw) * phase_hg(s.g, mu);
-            factor = c.l.lsun * ph / s.cone_pdf;
+            factor = lsun * ph / s.cone_pdf;
         }
-        ray_setup(s, c.px + (double)ld.x * kRayEpsilonKm, c.py + (double)ld.y * kRayEpsilonKm,
-                  c.pz + (double)ld.z * kRayEpsilonKm, ld, c.ray);
-        if (!(c.ray.hit && c.ray.kind == 1)) {
+        bool hit = ray_begin(s, c, c.px + (double)ld.x * kRayEpsilonKm, c.py + (double)ld.y * kRayEpsilonKm,
+                             c.pz + (double)ld.z * kRayEpsilonKm, ld,
+                             (s.flags & kFlagRatio) ? kTrackRatio : kTrackDelta);
+        if (!(hit && c.kind == 1)) {
+            c.st = ST_EVENT;
             c.nee_i++;
             act = c.nee_i < n_light ? ACT_NEE : ACT_CONT;
             continue;
         }
         c.nee_factor = factor;
-        track_begin(s, c, (s.flags & kFlagRatio) ? kTrackRatio : kTrackDelta);
         act = ACT_TRACKING;
     }
 
@@ -317,7 +339,7 @@ __device__ __forceinline__ void handle_event(const DevScene &s, LaneCtx &c, floa
                     c.px += (double)nd.x * kRayEpsilonKm;
                     c.py += (double)nd.y * kRayEpsilonKm;
                     c.pz += (double)nd.z * kRayEpsilonKm;
-                    c.dir = nd;
+                    mdir = nd;
                     act = ACT_MAIN;
                 }
             }
@@ -330,12 +352,8 @@ __device__ __forceinline__ void handle_event(const DevScene &s, LaneCtx &c, floa
         if (!(s.max_depth && c.depth >= s.max_depth)) {
             if (c.depth >= s.watchdog_limit) {
                 wd = true;
-            } else {
-                ray_setup(s, c.px, c.py, c.pz, c.dir, c.ray);
-                if (c.ray.hit) {
-                    track_begin(s, c, kTrackCollision);
-                    act = ACT_TRACKING;
-                }
+            } else if (ray_begin(s, c, c.px, c.py, c.pz, mdir, kTrackCollision)) {
+                act = ACT_TRACKING;
             }
         }
     }
@@ -360,11 +378,15 @@ __device__ __forceinline__ void start_path(const DevScene &s, LaneCtx &c, uint32
     c.slot = rem;
     uint32_t pixel = p0 + pl;
     uint32_t x = pixel % s.width, y = pixel / s.width;
-    c.l = s.lam[c.band];
+    const DevLambda l = s.lam[c.band];
+    c.br = l.br;
+    c.bo = l.bo;
+    c.bas = l.bas;
+    c.baa = l.baa;
     c.rng = rng_make(pixel, s0 + si, c.band, s);
     float j1 = rng_next(c.rng);
     float j2 = rng_next(c.rng);
-    c.dir = camera_dir(s, x, y, j1, j2);
+    c.vin = camera_dir(s, x, y, j1, j2);
     c.px = 0.0;
     c.py = s.obs_y;
     c.pz = 0.0;
@@ -375,7 +397,22 @@ __device__ __forceinline__ void start_path(const DevScene &s, LaneCtx &c, uint32
     c.ev = EV_START;
 }
 
-__global__ __launch_bounds__(256) void k_wavepool(DevScene s, WorkDesc w)
+// Item i of the launch belongs to the job j with jobs[j].item_base <= i <
+// jobs[j+1].item_base (jobs are stored in launch order, largest first).
+__device__ __forceinline__ uint32_t find_job(const PoolArgs &a, uint32_t item)
+{
+    uint32_t lo = 0, hi = a.n_jobs;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (a.jobs[mid].item_base <= item)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void wavepool_body(const PoolArgs &a)
 {
     __shared__ float lds_sum[kWavesPerBlock][kSlotsPerWave];
     __shared__ float lds_sq[kWavesPerBlock][kSlotsPerWave];
@@ -383,19 +420,25 @@ __global__ __launch_bounds__(256) void k_wavepool(DevScene s, WorkDesc w)
     const uint32_t lane = lane_id();
     float *acc_sum = lds_sum[wid];
     float *acc_sq = lds_sq[wid];
-    const uint32_t npx = s.width * s.height;
     bool wd = false;
+    uint32_t *diag = nullptr;
     LaneCtx c;
     for (;;) {
         uint32_t item = 0;
-        if (lane == 0) item = atomicAdd(w.counter, 1u);
+        if (lane == 0) item = atomicAdd(a.counter, 1u);
         item = __builtin_amdgcn_readfirstlane(item);
-        if (item >= w.n_items) break;
-        const uint32_t g = item % w.groups, ch = item / w.groups;
-        const uint32_t p0 = g * w.pixels_per_item;
-        const uint32_t np = min(w.pixels_per_item, npx - p0);
-        const uint32_t s0 = s.sample_offset + ch * w.samples_per_chunk;
-        const uint32_t ns = min(w.samples_per_chunk, s.spp - ch * w.samples_per_chunk);
+        if (item >= a.total_items) break;
+        if (item == 0 && lane == 0) *a.t_start = wall_clock64();
+        const DevJob &J = a.jobs[find_job(a, item)];
+        const DevScene &s = J.s;
+        diag = s.diag;
+        const uint32_t npx = s.width * s.height;
+        const uint32_t local = item - J.item_base;
+        const uint32_t g = local % J.groups, ch = local / J.groups;
+        const uint32_t p0 = g * J.pixels_per_item;
+        const uint32_t np = min(J.pixels_per_item, npx - p0);
+        const uint32_t s0 = s.sample_offset + ch * J.samples_per_chunk;
+        const uint32_t ns = min(J.samples_per_chunk, s.spp - ch * J.samples_per_chunk);
         const uint32_t nslots = np * s.nlam;
         const uint32_t total = nslots * ns;
         for (uint32_t i = lane; i < nslots; i += 64) {
@@ -421,27 +464,45 @@ __global__ __launch_bounds__(256) void k_wavepool(DevScene s, WorkDesc w)
             uint64_t evm = ballot(c.st == ST_EVENT);
             uint64_t trm = ballot(c.st == ST_TRACK);
             if (!evm && !trm) break;
-            if (evm && ((uint32_t)__popcll(evm) >= w.event_threshold || !trm)) {
+            if (evm && ((uint32_t)__popcll(evm) >= a.event_threshold || !trm)) {
                 if (c.st == ST_EVENT) handle_event(s, c, acc_sum, acc_sq, wd);
             }
-            for (uint32_t i = 0; i < w.track_steps; ++i) {
+            for (uint32_t i = 0; i < a.track_steps; ++i) {
                 if (!ballot(c.st == ST_TRACK)) break;
                 if (c.st == ST_TRACK) track_step(s, c, wd);
             }
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        float2 *dst = w.accum + ((size_t)ch * npx + p0) * s.nlam;
+        float2 *dst = J.accum + ((size_t)ch * npx + p0) * s.nlam;
         for (uint32_t i = lane; i < nslots; i += 64) dst[i] = make_float2(acc_sum[i], acc_sq[i]);
         __builtin_amdgcn_wave_barrier();
+        if (lane == 0) {
+            __threadfence();
+            uint32_t done = atomicAdd(J.done, 1u);
+            if (done + 1 == J.n_items) *J.t_done = wall_clock64();
+        }
     }
-    if (wd) atomicOr(s.diag, 1u);
+    if (wd && diag) atomicOr(diag, 1u);
 }
 
-// Reduces the chunk partials in chunk order, normalises by 1/spp and clamps
-// (integrator.rs:307-310), and forms XYZ = sum_band film * cmf * weight.
-__global__ __launch_bounds__(256) void k_finalize(DevScene s, const float2 *accum, uint32_t chunks)
+// The same integrator under two register budgets: the compiler's own
+// allocation (3 waves/SIMD) and a 128-VGPR cap (4 waves/SIMD, with a few
+// spills in the event handler).  The autotuner picks one per workload.
+__global__ __launch_bounds__(256) void k_wavepool(PoolArgs a) { wavepool_body(a); }
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_wavepool_o4(PoolArgs a)
 {
+    wavepool_body(a);
+}
+
+// Per job (blockIdx.y): reduces the chunk partials in chunk order,
+// normalises by 1/spp and clamps (integrator.rs:307-310), and forms
+// XYZ = sum_band film * cmf * weight.
+__global__ __launch_bounds__(256) void k_finalize(const DevJob *jobs)
+{
+    const DevJob &J = jobs[blockIdx.y];
+    const DevScene &s = J.s;
     uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t npx = s.width * s.height;
     if (pixel >= npx) return;
@@ -449,8 +510,8 @@ __global__ __launch_bounds__(256) void k_finalize(DevScene s, const float2 *accu
     float X = 0.0f, Y = 0.0f, Z = 0.0f;
     for (uint32_t b = 0; b < s.nlam; ++b) {
         float sum = 0.0f, sq = 0.0f;
-        for (uint32_t ch = 0; ch < chunks; ++ch) {
-            float2 v = accum[((size_t)ch * npx + pixel) * s.nlam + b];
+        for (uint32_t ch = 0; ch < J.chunks; ++ch) {
+            float2 v = J.accum[((size_t)ch * npx + pixel) * s.nlam + b];
             sum += v.x;
             sq += v.y;
         }
@@ -469,59 +530,85 @@ __global__ __launch_bounds__(256) void k_finalize(DevScene s, const float2 *accu
     }
 }
 
+// Fills n words with zero: used instead of a memset node so that the launch
+// sequence can be captured in a HIP graph.
 __global__ void k_zero_u32(uint32_t *p, uint32_t n)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
 }
 
-WavePlan plan_wavepool(const DevScene &s, const sky_tuning &t, int num_cus)
+void plan_job(const DevScene &s, const sky_tuning &t, int num_cus, uint32_t min_items, DevJob &j)
 {
-    WavePlan p;
     const uint32_t npx = s.width * s.height;
     uint32_t cap = kSlotsPerWave / s.nlam;
     uint32_t want = t.pixels_per_item ? t.pixels_per_item : (128u / s.nlam ? 128u / s.nlam : 1u);
-    p.pixels_per_item = want < 1 ? 1 : (want > cap ? cap : want);
-    p.groups = (npx + p.pixels_per_item - 1) / p.pixels_per_item;
-    uint32_t wpc = t.waves_per_cu ? t.waves_per_cu : 16u;
-    uint64_t waves = (uint64_t)(num_cus > 0 ? num_cus : 256) * wpc;
-    // enough items to balance the persistent waves, but >= 16 samples per item
-    uint64_t chunks = (4 * waves + p.groups - 1) / p.groups;
+    j.pixels_per_item = want < 1 ? 1 : (want > cap ? cap : want);
+    j.groups = (npx + j.pixels_per_item - 1) / j.pixels_per_item;
+    // split samples only as far as needed for min_items, keeping >= 16 samples per item
+    uint64_t chunks = ((uint64_t)min_items + j.groups - 1) / j.groups;
     uint64_t max_chunks = s.spp / 16 ? s.spp / 16 : 1;
     if (chunks > max_chunks) chunks = max_chunks;
     if (chunks < 1) chunks = 1;
-    p.samples_per_chunk = (uint32_t)((s.spp + chunks - 1) / chunks);
-    p.chunks = (s.spp + p.samples_per_chunk - 1) / p.samples_per_chunk;
-    p.n_items = p.groups * p.chunks;
-    uint64_t w = waves < p.n_items ? waves : p.n_items;
+    (void)num_cus;
+    j.samples_per_chunk = (uint32_t)((s.spp + chunks - 1) / chunks);
+    j.chunks = (s.spp + j.samples_per_chunk - 1) / j.samples_per_chunk;
+    j.n_items = j.groups * j.chunks;
+}
+
+size_t job_accum_bytes(const DevJob &j)
+{
+    return (size_t)j.chunks * j.s.width * j.s.height * j.s.nlam * sizeof(float2);
+}
+
+WavePlan plan_launch(const sky_tuning &t, int num_cus, uint32_t total_items)
+{
+    WavePlan p;
+    uint32_t wpc = t.waves_per_cu ? t.waves_per_cu : 16u;
+    uint64_t waves = (uint64_t)(num_cus > 0 ? num_cus : 256) * wpc;
+    uint64_t w = waves < total_items ? waves : total_items;
     p.block = 64 * kWavesPerBlock;
     p.grid = (uint32_t)((w + kWavesPerBlock - 1) / kWavesPerBlock);
-    p.accum_bytes = (size_t)p.chunks * npx * s.nlam * sizeof(float2);
+    p.n_items = total_items;
     p.event_threshold = t.event_threshold ? t.event_threshold : 32u;
     p.track_steps = t.track_steps ? t.track_steps : 8u;
+    p.occupancy = t.occupancy;
     return p;
 }
 
-hipError_t launch_wavepool(const DevScene &s, const WavePlan &p, void *accum, uint32_t *counter,
-                           hipStream_t stream)
+uint32_t launch_min_items(const sky_tuning &t, int num_cus)
 {
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, counter, 1u);
-    WorkDesc w;
-    w.pixels_per_item = p.pixels_per_item;
-    w.groups = p.groups;
-    w.chunks = p.chunks;
-    w.samples_per_chunk = p.samples_per_chunk;
-    w.n_items = p.n_items;
-    w.event_threshold = p.event_threshold;
-    w.track_steps = p.track_steps;
-    w.accum = (float2 *)accum;
-    w.counter = counter;
-    hipLaunchKernelGGL(k_wavepool, dim3(p.grid), dim3(p.block), 0, stream, s, w);
+    uint32_t wpc = t.waves_per_cu ? t.waves_per_cu : 16u;
+    return 4u * (uint32_t)(num_cus > 0 ? num_cus : 256) * wpc;
+}
+
+// meta: [0] item counter, [1..2] launch timestamp, [4 + 2 j ..] per-job done
+// counter / pad, then per-job finish timestamps; zeroed by the first kernel.
+hipError_t launch_pool(const DevJob *d_jobs, uint32_t n_jobs, uint32_t max_npx, const WavePlan &p,
+                       uint32_t *meta, uint32_t meta_words, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_zero_u32, dim3((meta_words + 255) / 256), dim3(256), 0, stream, meta, meta_words);
+    PoolArgs a;
+    a.jobs = d_jobs;
+    a.n_jobs = n_jobs;
+    a.total_items = p.n_items;
+    a.counter = meta;
+    a.t_start = (unsigned long long *)(meta + 2);
+    a.event_threshold = p.event_threshold;
+    a.track_steps = p.track_steps;
+    if (p.occupancy == 4)
+        hipLaunchKernelGGL(k_wavepool_o4, dim3(p.grid), dim3(p.block), 0, stream, a);
+    else
+        hipLaunchKernelGGL(k_wavepool, dim3(p.grid), dim3(p.block), 0, stream, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    uint32_t npx = s.width * s.height;
-    hipLaunchKernelGGL(k_finalize, dim3((npx + 255) / 256), dim3(256), 0, stream, s,
-                       (const float2 *)accum, p.chunks);
+    hipLaunchKernelGGL(k_finalize, dim3((max_npx + 255) / 256, n_jobs), dim3(256), 0, stream, d_jobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero(uint32_t *p, uint32_t n, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, stream, p, n);
     return hipGetLastError();
 }
 

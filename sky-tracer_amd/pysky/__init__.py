@@ -28,3 +28,4 @@ from .api import (  # noqa: F401
     render,
     wavelength_table,
 )
+from .batch import BatchResult, ragged_batch, render_batch  # noqa: F401

@@ -103,8 +103,37 @@ class Tuning(C.Structure):
         ("pixels_per_item", C.c_uint32),
         ("event_threshold", C.c_uint32),
         ("track_steps", C.c_uint32),
+        ("occupancy", C.c_uint32),
     ]
 
+
+class BatchItem(C.Structure):
+    _fields_ = [
+        ("planet", C.POINTER(Planet)),
+        ("atm", C.POINTER(Atmosphere)),
+        ("wl", C.POINTER(Wavelengths)),
+        ("cfg", C.POINTER(RenderConfigC)),
+        ("xyz", C.POINTER(C.c_float)),
+        ("film", C.POINTER(C.c_float)),
+    ]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [
+        ("kernel_ms", C.c_double),
+        ("total_ms", C.c_double),
+        ("path_samples", C.c_uint64),
+        ("renders", C.c_uint32),
+        ("items", C.c_uint32),
+        ("watchdog", C.c_uint32),
+        ("graph_reused", C.c_uint32),
+        ("latency_p50_ms", C.c_double),
+        ("latency_p99_ms", C.c_double),
+        ("latency_max_ms", C.c_double),
+    ]
+
+
+BATCH_NO_GRAPH = 1
 
 _lib = None
 
@@ -139,6 +168,8 @@ def lib() -> C.CDLL:
         "sky_render_device": ([ctxp, P, A, W, R, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                C.POINTER(RenderStats)], C.c_int),
         "sky_optical_depth": ([ctxp, P, A, W, dp, C.c_uint32, dp], C.c_int),
+        "sky_render_batch": ([ctxp, C.POINTER(BatchItem), C.c_uint32, C.c_uint32, dp, C.POINTER(BatchStats)],
+                             C.c_int),
         "sky_last_error": ([], C.c_char_p),
         "sky_version": ([], C.c_char_p),
     }

@@ -306,8 +306,8 @@ class Context:
     def handle(self) -> C.c_void_p:
         return self._h
 
-    def set_tuning(self, waves_per_cu=0, pixels_per_item=0, event_threshold=0, track_steps=0):
-        t = L.Tuning(waves_per_cu, pixels_per_item, event_threshold, track_steps)
+    def set_tuning(self, waves_per_cu=0, pixels_per_item=0, event_threshold=0, track_steps=0, occupancy=0):
+        t = L.Tuning(waves_per_cu, pixels_per_item, event_threshold, track_steps, occupancy)
         check(L.lib().sky_context_set_tuning(self._h, C.byref(t)))
 
     def close(self):
