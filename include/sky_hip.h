@@ -100,6 +100,10 @@ typedef struct sky_render_config {
     uint32_t majorant_layers;      /* 64 in the reference (data.rs:48) */
     uint32_t flags;                /* SKY_FLAG_*                       */
     double majorant_margin;        /* 1.05 in the reference            */
+    /* Optional host table [count * majorant_layers] (km^-1) replacing the
+     * computed majorant grid, e.g. to hold it fixed while parameters move
+     * (finite differences with common random numbers, the adjoint). */
+    const float *majorant_table;
 } sky_render_config;
 
 typedef struct sky_render_stats {
@@ -163,6 +167,25 @@ sky_status sky_render(sky_context *ctx, const sky_planet *planet, const sky_atmo
 sky_status sky_render_device(sky_context *ctx, const sky_planet *planet, const sky_atmosphere *atm,
                              const sky_wavelengths *wl, const sky_render_config *cfg, float *d_film,
                              float *d_film_m2, float *d_xyz, void *stream, sky_render_stats *stats);
+
+/* ---- adjoint: gradient of the rendered XYZ image ----
+ * Returns grad[k] = d/dtheta_k sum_{pixel,c} dl_dxyz[pixel*3+c] * XYZ[pixel*3+c]
+ * for the ten parameters, in this order: rayleigh_sca_550_m_inv,
+ * rayleigh_h_km, ozone_peak_m_inv, ozone_center_km, aerosol_sca_550_m_inv,
+ * aerosol_abs_550_m_inv, aerosol_h_km, aerosol_g, angstrom, ground_albedo.
+ * The paths of sky_render with the same configuration are replayed (same
+ * RNG keys) with the majorant held fixed; each path carries the
+ * log-derivative (score) of its sampling decisions and contribution weights.
+ * No counterpart in the reference, which has no derivatives. */
+#define SKY_NUM_PARAMS 10
+sky_status sky_render_adjoint(sky_context *ctx, const sky_planet *planet, const sky_atmosphere *atm,
+                              const sky_wavelengths *wl, const sky_render_config *cfg,
+                              const float *dl_dxyz, double *grad, sky_render_stats *stats);
+/* Same with dl_dxyz in device memory (ordered on `stream`); grad is host. */
+sky_status sky_render_adjoint_device(sky_context *ctx, const sky_planet *planet,
+                                     const sky_atmosphere *atm, const sky_wavelengths *wl,
+                                     const sky_render_config *cfg, const float *d_dl_dxyz,
+                                     double *grad, void *stream, sky_render_stats *stats);
 
 /* ---- ragged batch: many renders in one persistent launch ----
  * The reference renders one image per process and device

@@ -57,7 +57,7 @@ struct sky_context {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     sky_tuning tuning{};
-    DevBuf lam, maj, film, film_m2, xyz, accum, meta, diag, rays, tau, lamd, jobs;
+    DevBuf lam, maj, film, film_m2, xyz, accum, meta, diag, rays, tau, lamd, jobs, adj_units, adj_dl, adj_grad;
     std::vector<sky::DevJob> host_jobs;  // staging, alive until the stream syncs
     // ragged batches
     DevBuf b_lam, b_maj, b_xyz, b_film, b_accum, b_meta, b_jobs;
@@ -242,7 +242,7 @@ void sky_context_destroy(sky_context *c)
     if (c->graph) (void)hipGraphDestroy(c->graph);
     for (DevBuf *b : {&c->lam, &c->maj, &c->film, &c->film_m2, &c->xyz, &c->accum, &c->meta, &c->diag,
                       &c->rays, &c->tau, &c->lamd, &c->jobs, &c->b_lam, &c->b_maj, &c->b_xyz, &c->b_film,
-                      &c->b_accum, &c->b_meta, &c->b_jobs})
+                      &c->b_accum, &c->b_meta, &c->b_jobs, &c->adj_units, &c->adj_dl, &c->adj_grad})
         b->release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -366,6 +366,129 @@ sky_status sky_render(sky_context *ctx, const sky_planet *planet, const sky_atmo
     if (st_copy != SKY_OK) return st_copy;
     if (st == SKY_ERR_WATCHDOG) return fail(SKY_ERR_WATCHDOG, "GPU path watchdog was reached");
     return SKY_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Replay of the render's paths with the score bookkeeping (k_wavepool_adjoint).
+sky_status adjoint_impl(sky_context *ctx, const sky_planet *planet, const sky_atmosphere *atm,
+                        const sky_wavelengths *wl, const sky_render_config *cfg, const float *d_dl,
+                        double *grad, hipStream_t stream, sky_render_stats *stats)
+{
+    sky::HostScene hs;
+    std::string err;
+    sky_status st = sky::build_scene(planet, atm, wl, cfg, hs, err);
+    if (st != SKY_OK) return fail(st, err);
+    sky::DevScene &s = hs.dev;
+    HIP_TRY(ctx->lam.ensure(hs.lam.size() * sizeof(sky::DevLambda)), "alloc lambda table", SKY_ERR_OOM);
+    HIP_TRY(ctx->maj.ensure(hs.maj.size() * sizeof(float)), "alloc majorants", SKY_ERR_OOM);
+    HIP_TRY(ctx->diag.ensure(64), "alloc diagnostics", SKY_ERR_OOM);
+    HIP_TRY(ctx->meta.ensure(64), "alloc work counters", SKY_ERR_OOM);
+    HIP_TRY(ctx->jobs.ensure(sizeof(sky::DevJob)), "alloc job table", SKY_ERR_OOM);
+    HIP_TRY(ctx->adj_units.ensure(wl->count * sizeof(float4)), "alloc adjoint table", SKY_ERR_OOM);
+    HIP_TRY(ctx->adj_grad.ensure(SKY_NUM_PARAMS * sizeof(double)), "alloc gradient", SKY_ERR_OOM);
+    std::vector<float4> units(wl->count);
+    for (uint32_t i = 0; i < wl->count; ++i) {
+        double l = wl->center_nm[i], r = 550.0 / l, dl = (l - atm->ozone_lambda0_nm) / atm->ozone_sigma_nm;
+        units[i] = make_float4((float)(1000.0 * r * r * r * r), (float)(1000.0 * std::exp(-0.5 * dl * dl)),
+                               (float)(1000.0 * std::pow(l / 550.0, -atm->angstrom)), (float)std::log(l / 550.0));
+    }
+    HIP_TRY(hipMemcpyAsync(ctx->lam.p, hs.lam.data(), hs.lam.size() * sizeof(sky::DevLambda),
+                           hipMemcpyHostToDevice, stream), "upload lambda table", SKY_ERR_COPY);
+    HIP_TRY(hipMemcpyAsync(ctx->maj.p, hs.maj.data(), hs.maj.size() * sizeof(float), hipMemcpyHostToDevice,
+                           stream), "upload majorants", SKY_ERR_COPY);
+    HIP_TRY(hipMemcpyAsync(ctx->adj_units.p, units.data(), units.size() * sizeof(float4), hipMemcpyHostToDevice,
+                           stream), "upload adjoint table", SKY_ERR_COPY);
+    HIP_TRY(sky::launch_zero((uint32_t *)ctx->diag.p, 16, stream), "clear diagnostics", SKY_ERR_SYNC);
+    HIP_TRY(sky::launch_zero((uint32_t *)ctx->adj_grad.p, 2 * SKY_NUM_PARAMS, stream), "clear gradient",
+            SKY_ERR_SYNC);
+    s.lam = (const sky::DevLambda *)ctx->lam.p;
+    s.maj = (const float *)ctx->maj.p;
+    s.diag = (uint32_t *)ctx->diag.p;
+    s.film = s.film_m2 = s.xyz = nullptr;
+    ctx->host_jobs.assign(1, sky::DevJob{});
+    sky::DevJob &j = ctx->host_jobs[0];
+    j.s = s;
+    sky::plan_job(s, ctx->tuning, ctx->num_cus, sky::launch_min_items(ctx->tuning, ctx->num_cus), j);
+    j.item_base = 0;
+    HIP_TRY(ctx->accum.ensure(sky::job_accum_bytes(j)), "alloc partial sums", SKY_ERR_OOM);
+    j.accum = (float2 *)ctx->accum.p;
+    j.done = (uint32_t *)ctx->meta.p + 4;
+    j.t_done = (unsigned long long *)((uint32_t *)ctx->meta.p + 6);
+    HIP_TRY(hipMemcpyAsync(ctx->jobs.p, &j, sizeof(j), hipMemcpyHostToDevice, stream), "upload job", SKY_ERR_COPY);
+    sky::WavePlan plan = sky::plan_launch(ctx->tuning, ctx->num_cus, j.n_items);
+    sky::AdjArgs A;
+    A.dl_dxyz = d_dl;
+    A.lam_units = (const float4 *)ctx->adj_units.p;
+    A.inv_hr2 = (float)(1.0 / (atm->rayleigh_h_km * atm->rayleigh_h_km));
+    A.inv_ha2 = (float)(1.0 / (atm->aerosol_h_km * atm->aerosol_h_km));
+    A.inv_spp = 1.0f / (float)s.spp;
+    A.grad = (double *)ctx->adj_grad.p;
+    HIP_TRY(hipEventRecord(ctx->ev0, stream), "event record", SKY_ERR_SYNC);
+    HIP_TRY(sky::launch_adjoint((const sky::DevJob *)ctx->jobs.p, plan, (uint32_t *)ctx->meta.p, 8, A, stream),
+            "launch adjoint", SKY_ERR_SYNC);
+    HIP_TRY(hipEventRecord(ctx->ev1, stream), "event record", SKY_ERR_SYNC);
+    uint32_t wd = 0;
+    HIP_TRY(hipMemcpyAsync(grad, ctx->adj_grad.p, SKY_NUM_PARAMS * sizeof(double), hipMemcpyDeviceToHost, stream),
+            "copy gradient", SKY_ERR_COPY);
+    HIP_TRY(hipMemcpyAsync(&wd, ctx->diag.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream), "read diagnostics",
+            SKY_ERR_COPY);
+    HIP_TRY(hipStreamSynchronize(stream), "synchronize", SKY_ERR_SYNC);
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    if (stats) {
+        stats->kernel_ms = ms;
+        stats->path_samples = (uint64_t)s.width * s.height * s.spp * s.nlam;
+        stats->watchdog = wd;
+        stats->workgroups = plan.grid;
+        stats->block = plan.block;
+        stats->items = plan.n_items;
+    }
+    if (wd) return fail(SKY_ERR_WATCHDOG, "GPU path watchdog was reached");
+    return SKY_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+sky_status sky_render_adjoint(sky_context *ctx, const sky_planet *planet, const sky_atmosphere *atm,
+                              const sky_wavelengths *wl, const sky_render_config *cfg, const float *dl_dxyz,
+                              double *grad, sky_render_stats *stats)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    sky_status st = sky_validate_config(planet, atm, wl, cfg);
+    if (st != SKY_OK) return st;
+    if (!dl_dxyz || !grad) return fail(SKY_ERR_INVALID_CONFIG, "null cotangent or gradient buffer");
+    if (!ctx) return fail(SKY_ERR_NO_DEVICE, "no device context");
+    HIP_TRY(hipSetDevice(ctx->device), "set device", SKY_ERR_DEVICE_INIT);
+    size_t bytes = (size_t)cfg->width * cfg->height * 3 * sizeof(float);
+    HIP_TRY(ctx->adj_dl.ensure(bytes), "alloc cotangent", SKY_ERR_OOM);
+    HIP_TRY(hipMemcpyAsync(ctx->adj_dl.p, dl_dxyz, bytes, hipMemcpyHostToDevice, ctx->stream), "upload cotangent",
+            SKY_ERR_COPY);
+    st = adjoint_impl(ctx, planet, atm, wl, cfg, (const float *)ctx->adj_dl.p, grad, ctx->stream, stats);
+    if (stats)
+        stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return st;
+}
+
+sky_status sky_render_adjoint_device(sky_context *ctx, const sky_planet *planet, const sky_atmosphere *atm,
+                                     const sky_wavelengths *wl, const sky_render_config *cfg,
+                                     const float *d_dl_dxyz, double *grad, void *stream, sky_render_stats *stats)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    sky_status st = sky_validate_config(planet, atm, wl, cfg);
+    if (st != SKY_OK) return st;
+    if (!d_dl_dxyz || !grad) return fail(SKY_ERR_INVALID_CONFIG, "null cotangent or gradient buffer");
+    if (!ctx) return fail(SKY_ERR_NO_DEVICE, "no device context");
+    HIP_TRY(hipSetDevice(ctx->device), "set device", SKY_ERR_DEVICE_INIT);
+    st = adjoint_impl(ctx, planet, atm, wl, cfg, d_dl_dxyz, grad, stream ? (hipStream_t)stream : ctx->stream,
+                      stats);
+    if (stats)
+        stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return st;
 }
 
 sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint32_t n, uint32_t flags,

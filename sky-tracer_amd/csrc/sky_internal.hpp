@@ -59,6 +59,15 @@ struct PoolArgs {
     uint32_t event_threshold, track_steps;
 };
 
+// Inputs of the adjoint replay (sky_render_adjoint).
+struct AdjArgs {
+    const float *dl_dxyz;     // [pixel * 3 + c]
+    const float4 *lam_units;  // per band: (beta_R, ozone, aerosol) per unit parameter, ln(l / 550)
+    float inv_hr2, inv_ha2;   // 1 / H^2
+    float inv_spp;
+    double *grad;             // [10], accumulated atomically
+};
+
 // Launch-wide settings of the wave-pool integrator (sky_wavepool.hip).
 struct WavePlan {
     uint32_t n_items = 0, grid = 0, block = 256, event_threshold = 32, track_steps = 8, occupancy = 0;
@@ -71,6 +80,8 @@ WavePlan plan_launch(const sky_tuning &t, int num_cus, uint32_t total_items);
 hipError_t launch_pool(const DevJob *d_jobs, uint32_t n_jobs, uint32_t max_npx, const WavePlan &p,
                        uint32_t *meta, uint32_t meta_words, hipStream_t stream);
 hipError_t launch_zero(uint32_t *p, uint32_t n, hipStream_t stream);
+hipError_t launch_adjoint(const DevJob *d_jobs, const WavePlan &p, uint32_t *meta, uint32_t meta_words,
+                          const AdjArgs &A, hipStream_t stream);
 hipError_t launch_optical_depth(const DevScene &s, const double *rays, uint32_t n,
                                 const double *dev_lambda_d, double *tau, hipStream_t stream);
 

@@ -190,7 +190,18 @@ sky_status build_scene(const sky_planet *planet, const sky_atmosphere *a, const 
 
     out.nlam = wl->count;
     fill_wavelength_rows(a, wl, out.lam);
-    fill_majorants(planet, a, wl, cfg->majorant_layers, cfg->majorant_margin, out.maj);
+    if (cfg->majorant_table) {
+        size_t n = (size_t)wl->count * cfg->majorant_layers;
+        out.maj.assign(cfg->majorant_table, cfg->majorant_table + n);
+        for (float m : out.maj) {
+            if (!(m > 0.0f) || !std::isfinite(m)) {
+                err = "majorant table entries must be finite and positive";
+                return SKY_ERR_INVALID_CONFIG;
+            }
+        }
+    } else {
+        fill_majorants(planet, a, wl, cfg->majorant_layers, cfg->majorant_margin, out.maj);
+    }
 
     DevScene &s = out.dev;
     std::memset(&s, 0, sizeof(s));

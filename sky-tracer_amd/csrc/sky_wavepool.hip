@@ -67,6 +67,104 @@ struct LaneCtx {
     uint32_t guard;
 };
 
+// ------------------------------------------------------------ adjoint
+// The replay pass of sky_render_adjoint runs the same state machine with
+// ADJ = true.  With the majorant fixed, the law of a path depends on the
+// parameters only through the accept/reject tests (probability sigma_t / mu),
+// the albedo weight sigma_s / sigma_t, the phase function (sampled or used as
+// a weight), the ground albedo and the shadow-ray tests, so
+//   d E[sum_k C_k] = E[ sum_k C_k (d log C_k + S_k) ],
+// where S_k is the summed log-derivative of the decisions before C_k.  Each
+// lane carries S (main path), SS (current NEE sample: S plus its own terms)
+// and G (this path's gradient), all 10-vectors; Russian roulette is held
+// detached.
+constexpr int kNP = 10;
+
+struct AdjLane {
+    float S[kNP], SS[kNP], G[kNP];
+    float w;         // dLoss / dL of this path's (pixel, band), divided by spp
+    float vsr, vsas; // sigma_R and sigma_As at the scattering vertex
+    float vh;        // altitude of the scattering vertex
+};
+
+// partials of sigma_R, sigma_O and the two aerosol coefficients at altitude
+// h with respect to the ten parameters (sky_hip.h order).
+struct DSig {
+    float r0, r1;      // sigma_R: d/d beta_R, d/d H_R
+    float o2, o3;      // sigma_O: d/d peak, d/d centre
+    float as4, as6, as8;  // sigma_As: d/d beta_As, d/d H_A, d/d angstrom
+    float aa5, aa6, aa8;  // sigma_Aa: d/d beta_Aa, d/d H_A, d/d angstrom
+};
+
+__device__ __forceinline__ DSig dsig(const DevScene &s, const AdjArgs &A, uint32_t band, float h, float bo,
+                                     const Coeffs &co)
+{
+    DSig d{};
+    if (h < 0.0f || h > s.topf) return d;
+    const float4 u = A.lam_units[band];
+    float rr = expf(-h * s.inv_hr);
+    float ra = expf(-h * s.inv_ha);
+    float dh = h - s.hc;
+    float tent = fmaxf(1.0f - fabsf(dh) * s.inv_hw, 0.0f);
+    d.r0 = u.x * rr;
+    d.r1 = co.sr * h * A.inv_hr2;
+    d.o2 = u.y * tent;
+    d.o3 = fabsf(dh) * s.inv_hw < 1.0f ? bo * (dh > 0.0f ? s.inv_hw : -s.inv_hw) : 0.0f;
+    d.as4 = u.z * ra;
+    d.aa5 = u.z * ra;
+    d.as6 = co.sas * h * A.inv_ha2;
+    d.aa6 = co.saa * h * A.inv_ha2;
+    d.as8 = -co.sas * u.w;
+    d.aa8 = -co.saa * u.w;
+    return d;
+}
+
+// v += f * d sigma_t
+__device__ __forceinline__ void add_dsigma_t(float *v, const DSig &d, float f)
+{
+    v[0] += f * d.r0;
+    v[1] += f * d.r1;
+    v[2] += f * d.o2;
+    v[3] += f * d.o3;
+    v[4] += f * d.as4;
+    v[5] += f * d.aa5;
+    v[6] += f * (d.as6 + d.aa6);
+    v[8] += f * (d.as8 + d.aa8);
+}
+
+// v += f * d sigma_s
+__device__ __forceinline__ void add_dsigma_s(float *v, const DSig &d, float f)
+{
+    v[0] += f * d.r0;
+    v[1] += f * d.r1;
+    v[4] += f * d.as4;
+    v[6] += f * d.as6;
+    v[8] += f * d.as8;
+}
+
+// v += d log p(mu), p = (sigma_R p_R + sigma_As p_HG) / sigma_s at the vertex.
+__device__ __forceinline__ void add_dlog_phase(const DevScene &s, const AdjArgs &A, const AdjLane &a,
+                                               uint32_t band, float mu, float bo, float *v)
+{
+    Coeffs co{a.vsr, a.vsas, 0.0f, 0.0f};
+    // the aerosol absorption share of sigma_As's partials equals sigma_As's
+    DSig d = dsig(s, A, band, a.vh, bo, co);
+    float pr = phase_rayleigh(mu), ph = phase_hg(s.g, mu);
+    float num = a.vsr * pr + a.vsas * ph;
+    float scat = a.vsr + a.vsas;
+    if (!(num > 0.0f) || !(scat > 0.0f)) return;
+    float inum = 1.0f / num, iscat = 1.0f / scat;
+    float fr = pr * inum - iscat, fa = ph * inum - iscat;
+    v[0] += fr * d.r0;
+    v[1] += fr * d.r1;
+    v[4] += fa * d.as4;
+    v[6] += fa * d.as6;
+    v[8] += fa * d.as8;
+    float g = s.g, den = 1.0f + g * g - 2.0f * g * mu;
+    float dph = ph * (-2.0f * g / (1.0f - g * g) - 3.0f * (g - mu) / den);
+    v[7] += a.vsas * dph * inum;
+}
+
 __device__ __forceinline__ void seg_begin(const DevScene &s, LaneCtx &c)
 {
     Ray r;
@@ -123,7 +221,9 @@ __device__ __forceinline__ Coeffs lane_medium(const DevScene &s, const LaneCtx &
 // One tracking micro-step: at most one RNG draw.  An empty segment (its
 // end clamped to t) is crossed without a draw, as the reference's inner
 // `while t < segment_end` loop does.
-__device__ __forceinline__ void track_step(const DevScene &s, LaneCtx &c, bool &wd)
+template <bool ADJ>
+__device__ __forceinline__ void track_step(const DevScene &s, LaneCtx &c, AdjLane &a, const AdjArgs &A,
+                                           bool &wd)
 {
     const bool carried = (s.flags & kFlagCarriedTau) != 0;
     const bool empty = !(c.t < c.seg_end);
@@ -141,6 +241,14 @@ __device__ __forceinline__ void track_step(const DevScene &s, LaneCtx &c, bool &
                 c.ev = EV_SHADOW;
             }
         } else {
+            if constexpr (ADJ) {
+                // null collision: d log(1 - sigma_t / mu) = -d sigma_t / (mu - sigma_t)
+                Coeffs co = lane_medium(s, c, lane_altitude(s, c, c.t));
+                float st = co.sr + co.sas + co.so + co.saa;
+                float f = -1.0f / fmaxf(c.maj - st, 1e-30f);
+                DSig d = dsig(s, A, c.band, lane_altitude(s, c, c.t), c.bo, co);
+                add_dsigma_t(c.mode == kTrackCollision ? a.S : a.SS, d, f);
+            }
             c.aux = 1.0f;
         }
         return;
@@ -193,6 +301,11 @@ __device__ __forceinline__ void track_step(const DevScene &s, LaneCtx &c, bool &
     float ratio = fminf(fmaxf(st / c.maj, 0.0f), 1.0f);
     if (c.mode == kTrackRatio) {
         c.aux *= 1.0f - ratio;
+        if constexpr (ADJ) {
+            float f = -1.0f / fmaxf(c.maj - st, 1e-30f);
+            DSig d = dsig(s, A, c.band, lane_altitude(s, c, c.t), c.bo, co);
+            add_dsigma_t(a.SS, d, f);
+        }
     } else {
         c.pend = 1;
         c.aux = ratio;
@@ -212,8 +325,9 @@ __device__ __forceinline__ void advance_vertex(LaneCtx &c, float t)
 enum Act : int { ACT_NONE = 0, ACT_NEE = 1, ACT_CONT = 2, ACT_MAIN = 3, ACT_FINISH = 4, ACT_TRACKING = 5 };
 
 // Event handler: sky_trace.wgsl:703-776 between two tracking passes.
-__device__ __forceinline__ void handle_event(const DevScene &s, LaneCtx &c, float *acc_sum,
-                                             float *acc_sq, bool &wd)
+template <bool ADJ>
+__device__ __forceinline__ void handle_event(const DevScene &s, LaneCtx &c, AdjLane &a, const AdjArgs &A,
+                                             float *acc_sum, float *acc_sq, float *acc_grad, bool &wd)
 {
     const uint32_t n_light = s.dls ? s.dls : 1u;
     int act = ACT_NONE;
@@ -223,7 +337,13 @@ __device__ __forceinline__ void handle_event(const DevScene &s, LaneCtx &c, floa
     } else if (c.ev == EV_BOUNDARY) {
         if (c.kind == 1) {
             // exit: the sun disk is seen only by the camera ray (wgsl:745-749)
-            if (c.depth == 0 && in_sun_cone(s, c.d)) c.L = c.L + c.tp * s.lam[c.band].lsun;
+            if (c.depth == 0 && in_sun_cone(s, c.d)) {
+                float C = c.tp * s.lam[c.band].lsun;
+                c.L = c.L + C;
+                if constexpr (ADJ) {
+                    for (int k = 0; k < kNP; ++k) a.G[k] += a.w * C * a.S[k];
+                }
+            }
             act = ACT_FINISH;
         } else if (c.kind == 2) {
             act = ACT_FINISH;  // tracking watchdog
@@ -235,11 +355,19 @@ __device__ __forceinline__ void handle_event(const DevScene &s, LaneCtx &c, floa
             act = ACT_NEE;
         }
     } else if (c.ev == EV_HIT) {
-        Coeffs co = lane_medium(s, c, lane_altitude(s, c, c.t));
+        const float hv = lane_altitude(s, c, c.t);
+        Coeffs co = lane_medium(s, c, hv);
         advance_vertex(c, c.t);
         float scat = co.sr + co.sas;
         float ext = scat + co.so + co.saa;
         act = ACT_FINISH;
+        if constexpr (ADJ) {
+            // accepted collision times the albedo weight: d log(sigma_s / mu)
+            if (scat > 0.0f) add_dsigma_s(a.S, dsig(s, A, c.band, hv, c.bo, co), 1.0f / scat);
+            a.vsr = co.sr;
+            a.vsas = co.sas;
+            a.vh = hv;
+        }
         if (ext > 0.0f && scat > 0.0f) {
             c.tp = c.tp * fminf(fmaxf(scat / ext, 0.0f), 1.0f);
             if (c.tp > 0.0f) {
@@ -253,6 +381,11 @@ __device__ __forceinline__ void handle_event(const DevScene &s, LaneCtx &c, floa
         }
     } else {  // EV_SHADOW
         c.nee_sum += c.nee_factor * c.aux;
+        if constexpr (ADJ) {
+            float C = c.tp * c.nee_factor * c.aux / (float)n_light;
+            if (C != 0.0f)
+                for (int k = 0; k < kNP; ++k) a.G[k] += a.w * C * a.SS[k];
+        }
         c.nee_i++;
         act = c.nee_i < n_light ? ACT_NEE : ACT_CONT;
     }
@@ -268,6 +401,10 @@ __device__ __forceinline__ void handle_event(const DevScene &s, LaneCtx &c, floa
             double pn = 1.0 / sqrt(c.px * c.px + c.py * c.py + c.pz * c.pz);
             float cs = fmaxf((float)(c.px * pn) * ld.x + (float)(c.py * pn) * ld.y + (float)(c.pz * pn) * ld.z, 0.0f);
             factor = s.albedo * kInvPi * lsun * cs / s.cone_pdf;
+            if constexpr (ADJ) {
+                for (int k = 0; k < kNP; ++k) a.SS[k] = a.S[k];
+                if (s.albedo > 0.0f) a.SS[9] += 1.0f / s.albedo;
+            }
             if (!(cs > 0.0f)) {
                 c.nee_i++;
                 act = c.nee_i < n_light ? ACT_NEE : ACT_CONT;
@@ -277,6 +414,10 @@ __device__ __forceinline__ void handle_event(const DevScene &s, LaneCtx &c, floa
             float mu = dot3(c.vin, ld);
             float ph = c.rw * phase_rayleigh(mu) + (1.0f - c.rw) * phase_hg(s.g, mu);
             factor = lsun * ph / s.cone_pdf;
+            if constexpr (ADJ) {
+                for (int k = 0; k < kNP; ++k) a.SS[k] = a.S[k];
+                add_dlog_phase(s, A, a, c.band, mu, c.bo, a.SS);
+            }
         }
         bool hit = ray_begin(s, c, c.px + (double)ld.x * kRayEpsilonKm, c.py + (double)ld.y * kRayEpsilonKm,
                              c.pz + (double)ld.z * kRayEpsilonKm, ld,
@@ -305,6 +446,7 @@ __device__ __forceinline__ void handle_event(const DevScene &s, LaneCtx &c, floa
                 float st = sqrtf(fmaxf(1.0f - mu * mu, 0.0f));
                 nd = frame_dir(c.vin, mu, st, kTau * rng_next(c.rng));
                 w = 1.0f;
+                if constexpr (ADJ) add_dlog_phase(s, A, a, c.band, dot3(c.vin, nd), c.bo, a.S);
             } else {
                 float z = 1.0f - 2.0f * rng_next(c.rng);
                 float rr = sqrtf(fmaxf(1.0f - z * z, 0.0f));
@@ -314,6 +456,7 @@ __device__ __forceinline__ void handle_event(const DevScene &s, LaneCtx &c, floa
                 nd = f3{rr * cp, z, rr * sp};
                 float mu = dot3(c.vin, nd);
                 w = (rw * phase_rayleigh(mu) + (1.0f - rw) * phase_hg(s.g, mu)) / kIsotropicPdf;
+                if constexpr (ADJ) add_dlog_phase(s, A, a, c.band, mu, c.bo, a.S);
             }
         } else {
             double pn = 1.0 / sqrt(c.px * c.px + c.py * c.py + c.pz * c.pz);
@@ -327,6 +470,9 @@ __device__ __forceinline__ void handle_event(const DevScene &s, LaneCtx &c, floa
             float pdf = co_out * kInvPi;
             ok = pdf > 0.0f && co_out > 0.0f;
             w = ok ? s.albedo * kInvPi * co_out / pdf : 0.0f;
+            if constexpr (ADJ) {
+                if (s.albedo > 0.0f) a.S[9] += 1.0f / s.albedo;
+            }
         }
         act = ACT_FINISH;
         if (ok) {
@@ -334,6 +480,9 @@ __device__ __forceinline__ void handle_event(const DevScene &s, LaneCtx &c, floa
             if (roulette(s, c.tp, c.depth, c.rng)) {
                 if (c.L < 0.0f || c.L != c.L || c.tp < 0.0f || c.tp != c.tp) {
                     c.L = 0.0f;
+                    if constexpr (ADJ) {
+                        for (int k = 0; k < kNP; ++k) a.G[k] = 0.0f;
+                    }
                 } else {
                     c.depth++;
                     c.px += (double)nd.x * kRayEpsilonKm;
@@ -362,13 +511,18 @@ __device__ __forceinline__ void handle_event(const DevScene &s, LaneCtx &c, floa
         float Lf = fmaxf(c.L, 0.0f);
         atomicAdd(&acc_sum[c.slot], Lf);
         atomicAdd(&acc_sq[c.slot], Lf * Lf);
+        if constexpr (ADJ) {
+            for (int k = 0; k < kNP; ++k)
+                if (a.G[k] != 0.0f && a.G[k] == a.G[k]) atomicAdd(&acc_grad[k], a.G[k]);
+        }
         c.st = ST_NEED;
     }
 }
 
 // Path start (main, sky_trace.wgsl:799-807): triple j of the item.
-__device__ __forceinline__ void start_path(const DevScene &s, LaneCtx &c, uint32_t j, uint32_t p0,
-                                           uint32_t np, uint32_t s0)
+template <bool ADJ>
+__device__ __forceinline__ void start_path(const DevScene &s, LaneCtx &c, AdjLane &a, const AdjArgs &A,
+                                           uint32_t j, uint32_t p0, uint32_t np, uint32_t s0)
 {
     uint32_t per_sample = np * s.nlam;
     uint32_t si = j / per_sample;
@@ -395,6 +549,14 @@ __device__ __forceinline__ void start_path(const DevScene &s, LaneCtx &c, uint32
     c.depth = 0;
     c.st = ST_EVENT;
     c.ev = EV_START;
+    if constexpr (ADJ) {
+        const float *g = A.dl_dxyz + 3 * (size_t)pixel;
+        a.w = (g[0] * l.wx + g[1] * l.wy + g[2] * l.wz) * A.inv_spp;
+        for (int k = 0; k < kNP; ++k) {
+            a.S[k] = 0.0f;
+            a.G[k] = 0.0f;
+        }
+    }
 }
 
 // Item i of the launch belongs to the job j with jobs[j].item_base <= i <
@@ -412,17 +574,21 @@ __device__ __forceinline__ uint32_t find_job(const PoolArgs &a, uint32_t item)
     return lo;
 }
 
-__device__ __forceinline__ void wavepool_body(const PoolArgs &a)
+template <bool ADJ>
+__device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &A)
 {
     __shared__ float lds_sum[kWavesPerBlock][kSlotsPerWave];
     __shared__ float lds_sq[kWavesPerBlock][kSlotsPerWave];
+    __shared__ float lds_grad[kWavesPerBlock][ADJ ? 16 : 1];
     const uint32_t wid = threadIdx.x >> 6;
     const uint32_t lane = lane_id();
     float *acc_sum = lds_sum[wid];
     float *acc_sq = lds_sq[wid];
+    float *acc_grad = lds_grad[wid];
     bool wd = false;
     uint32_t *diag = nullptr;
     LaneCtx c;
+    AdjLane al;
     for (;;) {
         uint32_t item = 0;
         if (lane == 0) item = atomicAdd(a.counter, 1u);
@@ -445,6 +611,9 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a)
             acc_sum[i] = 0.0f;
             acc_sq[i] = 0.0f;
         }
+        if constexpr (ADJ) {
+            if (lane < kNP) acc_grad[lane] = 0.0f;
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         uint32_t next = 0;
@@ -455,7 +624,7 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a)
                 if (c.st == ST_NEED) {
                     uint32_t j = next + mbcnt(need);
                     if (j < total)
-                        start_path(s, c, j, p0, np, s0);
+                        start_path<ADJ>(s, c, al, A, j, p0, np, s0);
                     else
                         c.st = ST_IDLE;
                 }
@@ -465,17 +634,20 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a)
             uint64_t trm = ballot(c.st == ST_TRACK);
             if (!evm && !trm) break;
             if (evm && ((uint32_t)__popcll(evm) >= a.event_threshold || !trm)) {
-                if (c.st == ST_EVENT) handle_event(s, c, acc_sum, acc_sq, wd);
+                if (c.st == ST_EVENT) handle_event<ADJ>(s, c, al, A, acc_sum, acc_sq, acc_grad, wd);
             }
             for (uint32_t i = 0; i < a.track_steps; ++i) {
                 if (!ballot(c.st == ST_TRACK)) break;
-                if (c.st == ST_TRACK) track_step(s, c, wd);
+                if (c.st == ST_TRACK) track_step<ADJ>(s, c, al, A, wd);
             }
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         float2 *dst = J.accum + ((size_t)ch * npx + p0) * s.nlam;
         for (uint32_t i = lane; i < nslots; i += 64) dst[i] = make_float2(acc_sum[i], acc_sq[i]);
+        if constexpr (ADJ) {
+            if (lane < kNP) atomicAdd(&A.grad[lane], (double)acc_grad[lane]);
+        }
         __builtin_amdgcn_wave_barrier();
         if (lane == 0) {
             __threadfence();
@@ -489,12 +661,15 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a)
 // The same integrator under two register budgets: the compiler's own
 // allocation (3 waves/SIMD) and a 128-VGPR cap (4 waves/SIMD, with a few
 // spills in the event handler).  The autotuner picks one per workload.
-__global__ __launch_bounds__(256) void k_wavepool(PoolArgs a) { wavepool_body(a); }
+__global__ __launch_bounds__(256) void k_wavepool(PoolArgs a) { wavepool_body<false>(a, AdjArgs{}); }
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_wavepool_o4(PoolArgs a)
 {
-    wavepool_body(a);
+    wavepool_body<false>(a, AdjArgs{});
 }
+
+// Path-replay adjoint: the forward state machine plus the score bookkeeping.
+__global__ __launch_bounds__(256) void k_wavepool_adjoint(PoolArgs a, AdjArgs A) { wavepool_body<true>(a, A); }
 
 // Per job (blockIdx.y): reduces the chunk partials in chunk order,
 // normalises by 1/spp and clamps (integrator.rs:307-310), and forms
@@ -603,6 +778,22 @@ hipError_t launch_pool(const DevJob *d_jobs, uint32_t n_jobs, uint32_t max_npx, 
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_finalize, dim3((max_npx + 255) / 256, n_jobs), dim3(256), 0, stream, d_jobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint(const DevJob *d_jobs, const WavePlan &p, uint32_t *meta, uint32_t meta_words,
+                          const AdjArgs &A, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_zero_u32, dim3((meta_words + 255) / 256), dim3(256), 0, stream, meta, meta_words);
+    PoolArgs a;
+    a.jobs = d_jobs;
+    a.n_jobs = 1;
+    a.total_items = p.n_items;
+    a.counter = meta;
+    a.t_start = (unsigned long long *)(meta + 2);
+    a.event_threshold = p.event_threshold;
+    a.track_steps = p.track_steps;
+    hipLaunchKernelGGL(k_wavepool_adjoint, dim3(p.grid), dim3(p.block), 0, stream, a, A);
     return hipGetLastError();
 }
 

@@ -29,3 +29,4 @@ from .api import (  # noqa: F401
     wavelength_table,
 )
 from .batch import BatchResult, ragged_batch, render_batch  # noqa: F401
+from .adjoint import fixed_majorant, params_vector, render_adjoint, with_params  # noqa: F401

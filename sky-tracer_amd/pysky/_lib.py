@@ -82,6 +82,7 @@ class RenderConfigC(C.Structure):
         ("majorant_layers", C.c_uint32),
         ("flags", C.c_uint32),
         ("majorant_margin", C.c_double),
+        ("majorant_table", C.POINTER(C.c_float)),
     ]
 
 
@@ -168,6 +169,9 @@ def lib() -> C.CDLL:
         "sky_render_device": ([ctxp, P, A, W, R, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                C.POINTER(RenderStats)], C.c_int),
         "sky_optical_depth": ([ctxp, P, A, W, dp, C.c_uint32, dp], C.c_int),
+        "sky_render_adjoint": ([ctxp, P, A, W, R, fp, dp, C.POINTER(RenderStats)], C.c_int),
+        "sky_render_adjoint_device": ([ctxp, P, A, W, R, C.c_void_p, dp, C.c_void_p, C.POINTER(RenderStats)],
+                                      C.c_int),
         "sky_render_batch": ([ctxp, C.POINTER(BatchItem), C.c_uint32, C.c_uint32, dp, C.POINTER(BatchStats)],
                              C.c_int),
         "sky_last_error": ([], C.c_char_p),

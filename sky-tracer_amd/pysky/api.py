@@ -116,6 +116,8 @@ class RenderConfig:
     ratio_tracking: bool = False
     carried_tau: bool = False
     simple_kernel: bool = False
+    # optional fixed majorant grid [bands, layers] (km^-1)
+    majorant_table: Optional[np.ndarray] = None
 
     @property
     def flags(self) -> int:
@@ -248,6 +250,13 @@ def config_c(scene: SceneData, cfg: RenderConfig) -> L.RenderConfigC:
     c.majorant_layers = cfg.majorant_layers
     c.flags = cfg.flags
     c.majorant_margin = cfg.majorant_margin
+    if cfg.majorant_table is not None:
+        t = np.ascontiguousarray(cfg.majorant_table, dtype=np.float32).ravel()
+        if t.size != scene.band_count * cfg.majorant_layers:
+            raise InvalidConfig(f"majorant table has {t.size} entries, expected "
+                                f"{scene.band_count * cfg.majorant_layers}")
+        c.majorant_table = t.ctypes.data_as(C.POINTER(C.c_float))
+        c._keep = t
     return c
 
 
