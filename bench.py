@@ -98,7 +98,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--mode", default="carried_is", choices=sorted(MODES))
-    ap.add_argument("--layers", type=int, default=64)
+    ap.add_argument("--layers", type=int, default=0, help="majorant shells (0: tuned)")
+    ap.add_argument("--autotune", action="store_true", help="re-tune launch parameters first (untimed)")
     ap.add_argument("--spp", type=int, default=0, help="override samples per pixel")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -112,12 +113,21 @@ def main():
     import pysky
     from pysky.torch_ops import render_device
 
+    from pysky import autotune
+
     w = dict(WORKLOADS[args.workload])
     if args.spp:
         w["spp"] = args.spp
     scene = make_scene(w)
-    cfg = make_config(w, args.mode, args.layers)
+    cfg = make_config(w, args.mode, 64)
     ctx = pysky.Context(0)
+    if args.autotune:
+        autotune.autotune(scene, cfg, ctx, log=lambda m: print(m, file=sys.stderr, flush=True))
+    params = autotune.tuned(scene, cfg)
+    if args.layers:
+        params["majorant_layers"] = args.layers
+    cfg = autotune.apply(ctx, cfg, params)
+    args.layers = cfg.majorant_layers
     dev = torch.device("cuda:0")
     xyz = torch.zeros((w["height"], w["width"], 3), dtype=torch.float32, device=dev)
     samples = w["width"] * w["height"] * w["n_lambda"] * w["spp"]
@@ -172,7 +182,8 @@ def main():
         "data": "synthetic (analytic atmosphere, no external data)",
         "config": {"workload": f"{args.workload}: {w['width']}x{w['height']}, {w['n_lambda']} lambda, "
                                f"{w['spp']} spp, depth<={w['max_depth']}, sun {w['sun']} deg",
-                   "estimator": args.mode, "majorant_layers": args.layers, "parallelism": "1 GPU"},
+                   "estimator": args.mode, "majorant_layers": args.layers, "parallelism": "1 GPU",
+                   "tuning": params},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -165,20 +165,36 @@ __device__ __forceinline__ void add_dlog_phase(const DevScene &s, const AdjArgs 
     v[7] += a.vsas * dph * inum;
 }
 
-__device__ __forceinline__ void seg_begin(const DevScene &s, LaneCtx &c)
+// Majorant of (band, shell): from the wave's LDS copy of the job's grid when
+// it fits (nlam * L <= kMajLds), else from global memory.  A crossing needs
+// it immediately, so the LDS copy removes a global-load latency from every
+// shell crossing.
+constexpr int kMajLds = 1024;
+
+struct MajSrc {
+    const float *lds;  // wave's LDS copy or null
+};
+
+__device__ __forceinline__ float majorant(const DevScene &s, const MajSrc &m, uint32_t band, int k)
+{
+    uint32_t i = band * s.L + (uint32_t)k;
+    return m.lds ? m.lds[i] : s.maj[i];
+}
+
+__device__ __forceinline__ void seg_begin(const DevScene &s, const MajSrc &m, LaneCtx &c)
 {
     Ray r;
     r.q = c.q;
     r.bf = c.bf;
     float tc = next_crossing(s, r, c.k, c.out, c.kn, c.on);
     c.seg_end = fmaxf(fminf(tc, c.tmax), c.t);
-    c.maj = s.maj[c.band * s.L + c.k];
+    c.maj = majorant(s, m, c.band, c.k);
 }
 
 // Sets up a ray from (ox, oy, oz) along d (FP64 set-up, sky_device.hpp) and
 // starts tracking it; returns false if it has no boundary.
-__device__ __forceinline__ bool ray_begin(const DevScene &s, LaneCtx &c, double ox, double oy,
-                                          double oz, f3 d, int mode)
+__device__ __forceinline__ bool ray_begin(const DevScene &s, const MajSrc &m, LaneCtx &c, double ox,
+                                          double oy, double oz, f3 d, int mode)
 {
     Ray r;
     ray_setup(s, ox, oy, oz, d, r);
@@ -196,7 +212,7 @@ __device__ __forceinline__ bool ray_begin(const DevScene &s, LaneCtx &c, double 
     c.guard = 1;
     c.k = r.layer;
     c.out = r.outward;
-    seg_begin(s, c);
+    seg_begin(s, m, c);
     c.st = ST_TRACK;
     return r.hit != 0;
 }
@@ -222,8 +238,8 @@ __device__ __forceinline__ Coeffs lane_medium(const DevScene &s, const LaneCtx &
 // end clamped to t) is crossed without a draw, as the reference's inner
 // `while t < segment_end` loop does.
 template <bool ADJ>
-__device__ __forceinline__ void track_step(const DevScene &s, LaneCtx &c, AdjLane &a, const AdjArgs &A,
-                                           bool &wd)
+__device__ __forceinline__ void track_step(const DevScene &s, const MajSrc &m, LaneCtx &c, AdjLane &a,
+                                           const AdjArgs &A, bool &wd)
 {
     const bool carried = (s.flags & kFlagCarriedTau) != 0;
     const bool empty = !(c.t < c.seg_end);
@@ -292,7 +308,7 @@ __device__ __forceinline__ void track_step(const DevScene &s, LaneCtx &c, AdjLan
         c.guard++;
         c.k = c.kn;
         c.out = c.on;
-        seg_begin(s, c);
+        seg_begin(s, m, c);
         return;
     }
     c.t = tn;
@@ -326,8 +342,9 @@ enum Act : int { ACT_NONE = 0, ACT_NEE = 1, ACT_CONT = 2, ACT_MAIN = 3, ACT_FINI
 
 // Event handler: sky_trace.wgsl:703-776 between two tracking passes.
 template <bool ADJ>
-__device__ __forceinline__ void handle_event(const DevScene &s, LaneCtx &c, AdjLane &a, const AdjArgs &A,
-                                             float *acc_sum, float *acc_sq, float *acc_grad, bool &wd)
+__device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m, LaneCtx &c, AdjLane &a,
+                                             const AdjArgs &A, float *acc_sum, float *acc_sq, float *acc_grad,
+                                             bool &wd)
 {
     const uint32_t n_light = s.dls ? s.dls : 1u;
     int act = ACT_NONE;
@@ -419,7 +436,7 @@ __device__ __forceinline__ void handle_event(const DevScene &s, LaneCtx &c, AdjL
                 add_dlog_phase(s, A, a, c.band, mu, c.bo, a.SS);
             }
         }
-        bool hit = ray_begin(s, c, c.px + (double)ld.x * kRayEpsilonKm, c.py + (double)ld.y * kRayEpsilonKm,
+        bool hit = ray_begin(s, m, c, c.px + (double)ld.x * kRayEpsilonKm, c.py + (double)ld.y * kRayEpsilonKm,
                              c.pz + (double)ld.z * kRayEpsilonKm, ld,
                              (s.flags & kFlagRatio) ? kTrackRatio : kTrackDelta);
         if (!(hit && c.kind == 1)) {
@@ -501,7 +518,7 @@ __device__ __forceinline__ void handle_event(const DevScene &s, LaneCtx &c, AdjL
         if (!(s.max_depth && c.depth >= s.max_depth)) {
             if (c.depth >= s.watchdog_limit) {
                 wd = true;
-            } else if (ray_begin(s, c, c.px, c.py, c.pz, mdir, kTrackCollision)) {
+            } else if (ray_begin(s, m, c, c.px, c.py, c.pz, mdir, kTrackCollision)) {
                 act = ACT_TRACKING;
             }
         }
@@ -580,6 +597,8 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
     __shared__ float lds_sum[kWavesPerBlock][kSlotsPerWave];
     __shared__ float lds_sq[kWavesPerBlock][kSlotsPerWave];
     __shared__ float lds_grad[kWavesPerBlock][ADJ ? 16 : 1];
+    __shared__ float lds_maj[kWavesPerBlock][kMajLds];
+    const DevJob *loaded = nullptr;  // job whose majorants are in lds_maj
     const uint32_t wid = threadIdx.x >> 6;
     const uint32_t lane = lane_id();
     float *acc_sum = lds_sum[wid];
@@ -611,6 +630,15 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
             acc_sum[i] = 0.0f;
             acc_sq[i] = 0.0f;
         }
+        MajSrc msrc{nullptr};
+        const uint32_t nmaj = s.nlam * s.L;
+        if (nmaj <= (uint32_t)kMajLds) {
+            msrc.lds = lds_maj[wid];
+            if (loaded != &J) {
+                for (uint32_t i = lane; i < nmaj; i += 64) lds_maj[wid][i] = s.maj[i];
+                loaded = &J;
+            }
+        }
         if constexpr (ADJ) {
             if (lane < kNP) acc_grad[lane] = 0.0f;
         }
@@ -634,11 +662,11 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
             uint64_t trm = ballot(c.st == ST_TRACK);
             if (!evm && !trm) break;
             if (evm && ((uint32_t)__popcll(evm) >= a.event_threshold || !trm)) {
-                if (c.st == ST_EVENT) handle_event<ADJ>(s, c, al, A, acc_sum, acc_sq, acc_grad, wd);
+                if (c.st == ST_EVENT) handle_event<ADJ>(s, msrc, c, al, A, acc_sum, acc_sq, acc_grad, wd);
             }
             for (uint32_t i = 0; i < a.track_steps; ++i) {
                 if (!ballot(c.st == ST_TRACK)) break;
-                if (c.st == ST_TRACK) track_step<ADJ>(s, c, al, A, wd);
+                if (c.st == ST_TRACK) track_step<ADJ>(s, msrc, c, al, A, wd);
             }
         }
         __builtin_amdgcn_wave_barrier();

@@ -134,3 +134,16 @@ def test_film_api_mirrors_reference():
     assert f.as_array().shape == (2, 3, 4)
     assert pysky.OutputProjection.Panorama.label() == "panorama"
     assert pysky.OutputProjection.SkyViewLut.label() == "sky-view LUT"
+
+
+def test_cli_reports_invalid_config(tmp_path):
+    """sky_render (the sky-reference-render counterpart) validates before it
+    needs a device."""
+    import os
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(L.LIB_PATH), "..", "bin", "sky_render")
+    r = subprocess.run([exe, "--width", "0", "--height", "4", "--spp", "1", "--out", str(tmp_path)],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert "greater than zero" in r.stderr

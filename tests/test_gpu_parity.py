@@ -149,3 +149,22 @@ def test_optical_depth_matches_oracle(gpu_context):
     tau = pysky.optical_depth(scene, rays, context=gpu_context)
     ref = np.array([[O.optical_depth(op, b, r[:3], r[3:6], r[6]) for b in range(16)] for r in rays])
     np.testing.assert_allclose(tau, ref, rtol=1e-9, atol=1e-14)
+
+
+def test_cli_renders(tmp_path):
+    """sky_render CLI (main.rs:58-138 counterpart): writes the film, the XYZ
+    image and the manifest, and prints the timing line."""
+    import json
+    import os
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(pysky.LIB_PATH), "..", "bin", "sky_render")
+    r = subprocess.run([exe, "--width", "32", "--height", "16", "--spp", "4", "--sun-elevation-deg", "10",
+                        "--bands", "16", "--max-depth", "8", "--out", str(tmp_path)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "timing: load=" in r.stdout and "render=" in r.stdout
+    m = json.load(open(tmp_path / "asset.json"))
+    film = np.fromfile(tmp_path / "film_f32.raw", dtype=np.float32)
+    assert film.size == 32 * 16 * 16 and m["bands"] == 16
+    assert np.all(np.isfinite(film)) and np.any(film > 0)
