@@ -3,6 +3,13 @@ import sys
 
 import pytest
 
+try:
+    # torch's bundled HIP runtime must be loaded before libsky_hip.so pulls in
+    # /opt/rocm's, or torch later sees no device.
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 for p in (ROOT, os.path.join(ROOT, "oracle"), os.path.join(ROOT, "sky-tracer_amd")):
     if p not in sys.path:
