@@ -123,7 +123,7 @@ typedef struct sky_tuning {
     uint32_t pixels_per_item; /* pixels per wave work item (128 / count)       */
     uint32_t event_threshold; /* lanes waiting before events are handled (32)  */
     uint32_t track_steps;     /* tracking micro-steps between handler passes (8) */
-    uint32_t occupancy;       /* 0: compiler's register budget; 4: 128-VGPR variant */
+    uint32_t occupancy;       /* 0: compiler's register budget; 4, 5, 6, 8: at least that many waves per SIMD */
 } sky_tuning;
 
 /* ---- context ---- */

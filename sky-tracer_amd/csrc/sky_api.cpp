@@ -116,7 +116,7 @@ sky_status run_render(sky_context *ctx, sky::HostScene &hs, hipStream_t stream,
         j.t_done = (unsigned long long *)((uint32_t *)ctx->meta.p + 6);
         HIP_TRY(hipMemcpyAsync(ctx->jobs.p, &j, sizeof(j), hipMemcpyHostToDevice, stream), "upload job",
                 SKY_ERR_COPY);
-        plan = sky::plan_launch(ctx->tuning, ctx->num_cus, j.n_items);
+        plan = sky::plan_launch(ctx->tuning, ctx->num_cus, j.n_items, s.nlam * s.L);
     } else if (!s.film) {
         size_t bytes = (size_t)npx * s.nlam * sizeof(float);
         HIP_TRY(ctx->film.ensure(bytes), "alloc film", SKY_ERR_OOM);
@@ -254,7 +254,7 @@ sky_status sky_context_set_tuning(sky_context *c, const sky_tuning *t)
 {
     if (!c || !t) return fail(SKY_ERR_INVALID_CONFIG, "null context or tuning");
     if (t->event_threshold > 64 || t->pixels_per_item > 256 || t->waves_per_cu > 64 || t->track_steps > 4096 ||
-        (t->occupancy != 0 && t->occupancy != 4))
+        (t->occupancy != 0 && t->occupancy != 4 && t->occupancy != 5 && t->occupancy != 6 && t->occupancy != 8))
         return fail(SKY_ERR_INVALID_CONFIG, "tuning value out of range");
     c->tuning = *t;
     return SKY_OK;
@@ -418,7 +418,7 @@ sky_status adjoint_impl(sky_context *ctx, const sky_planet *planet, const sky_at
     j.done = (uint32_t *)ctx->meta.p + 4;
     j.t_done = (unsigned long long *)((uint32_t *)ctx->meta.p + 6);
     HIP_TRY(hipMemcpyAsync(ctx->jobs.p, &j, sizeof(j), hipMemcpyHostToDevice, stream), "upload job", SKY_ERR_COPY);
-    sky::WavePlan plan = sky::plan_launch(ctx->tuning, ctx->num_cus, j.n_items);
+    sky::WavePlan plan = sky::plan_launch(ctx->tuning, ctx->num_cus, j.n_items, s.nlam * s.L);
     sky::AdjArgs A;
     A.dl_dxyz = d_dl;
     A.lam_units = (const float4 *)ctx->adj_units.p;
@@ -520,7 +520,7 @@ sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint3
 
     // Table offsets.
     size_t n_lam = 0, n_maj = 0, n_xyz = 0, n_film = 0, n_acc = 0;
-    uint32_t max_npx = 0, total_items = 0;
+    uint32_t max_npx = 0, total_items = 0, max_maj = 0;
     std::vector<size_t> o_lam(n), o_maj(n), o_xyz(n), o_film(n), o_acc(n);
     ctx->host_jobs.assign(n, sky::DevJob{});
     const uint32_t min_items = 0;  // the batch balances across renders; no sample split needed
@@ -545,6 +545,7 @@ sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint3
         o_acc[i] = n_acc;
         n_acc += sky::job_accum_bytes(j) / sizeof(float2);
         max_npx = npx > max_npx ? npx : max_npx;
+        max_maj = h.maj.size() > max_maj ? (uint32_t)h.maj.size() : max_maj;
         samples += (uint64_t)npx * h.dev.spp * h.dev.nlam;
     }
     const uint32_t meta_words = 8 + 4 * n;
@@ -582,7 +583,7 @@ sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint3
     HIP_TRY(hipMemcpyAsync(ctx->b_jobs.p, ctx->host_jobs.data(), n * sizeof(sky::DevJob), hipMemcpyHostToDevice,
                            stream), "upload batch job table", SKY_ERR_COPY);
     HIP_TRY(sky::launch_zero((uint32_t *)ctx->diag.p, 16, stream), "clear diagnostics", SKY_ERR_SYNC);
-    sky::WavePlan plan = sky::plan_launch(ctx->tuning, ctx->num_cus, total_items);
+    sky::WavePlan plan = sky::plan_launch(ctx->tuning, ctx->num_cus, total_items, max_maj);
 
     // The graph holds only kernel nodes (zero, integrate, finalize); the
     // tables it reads are uploaded above on every call.
@@ -598,6 +599,7 @@ sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint3
     mix(plan.event_threshold);
     mix(plan.track_steps);
     mix(plan.occupancy);
+    mix(plan.lds_bytes);
     bool reused = false;
     HIP_TRY(hipEventRecord(ctx->ev0, stream), "event record", SKY_ERR_SYNC);
     if (use_graph) {

@@ -7,6 +7,11 @@
 // The altitude along a ray is r^2 - R^2 = q + (b + t)^2, which has no
 // cancellation at 6360 km (the reference's (r-R)(r+R) at sky_trace.wgsl:214 is
 // FP32 and loses ~0.5 m per operand there).
+//
+// FP32 transcendentals, reciprocals and roots use the gfx950 hardware
+// instructions (v_exp/v_log/v_sin/v_cos/v_rcp/v_rsq/v_sqrt_f32, about 1 ulp)
+// instead of the correctly rounded library sequences (10-40 instructions
+// each); FP64 roots are an FP32 root refined by one Newton step.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -16,7 +21,9 @@ namespace sky {
 
 constexpr float kPi = 3.14159265358979323846f;
 constexpr float kInvPi = 0.31830988618379067154f;
+constexpr float kInv2Pi = 0.15915494309189533577f;
 constexpr float kTau = 6.28318530717958647692f;
+constexpr float kLn2 = 0.69314718055994530942f;
 constexpr float kIsotropicPdf = 0.07957747154594766788f;  // sky_trace.wgsl:10
 constexpr float kRayEpsilonKm = 0.005f;                    // sky_trace.wgsl:8
 constexpr int kMaxLambda = 64;
@@ -32,13 +39,17 @@ struct DevLambda {
     float br, bo, bas, baa, lsun, wx, wy, wz;
 };
 
-// Everything one render needs on the device (kernel argument, by value).
+// Everything one render needs on the device.
 struct DevScene {
     double R, top, dz, delta_top, obs_y;
+    double inv_dz;     // 1 / dz
+    double kd1, kd2;   // shell k radius r_k: r_k^2 - R^2 = k kd1 + k^2 kd2 (2 R dz, dz^2)
     float Rf, R2f, topf, eps;
     float sun[3];
     float omc, cone_pdf;  // 1 - cos(sun radius) (= 2 sin^2(r/2)), 1/(2 pi omc)
     float inv_hr, inv_ha, hc, inv_hw, g, albedo;
+    float l2_hr, l2_ha;   // log2(e) / H: exp(-h / H) = exp2(-h l2_h)
+    float inv_w, inv_h;   // 1 / width, 1 / height
     // Sky-view LUT frame (sky_trace.wgsl:641-681), precomputed on the host.
     float svl_zha, svl_beta, svl_fw[3], svl_side[3];
     uint32_t width, height, spp, sample_offset;
@@ -53,6 +64,64 @@ struct DevScene {
     uint32_t *diag;    // [0] = watchdog flag
 };
 
+// ----------------------------------------------------------- fast math
+__device__ __forceinline__ float f_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float f_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float f_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float f_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float f_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ float f_log2(float x) { return __builtin_amdgcn_logf(x); }
+// sin / cos of 2 pi x: the hardware takes the angle in revolutions.
+__device__ __forceinline__ float f_sin_rev(float x) { return __builtin_amdgcn_sinf(x); }
+__device__ __forceinline__ float f_cos_rev(float x) { return __builtin_amdgcn_cosf(x); }
+
+// Free-flight optical depth -ln(max(1 - u, 1e-7)) (sky_trace.wgsl:424).
+__device__ __forceinline__ float free_flight(float u)
+{
+    return -kLn2 * f_log2(fmaxf(1.0f - u, 0.0000001f));
+}
+
+// sqrt(x) for x >= 0 in FP64: FP32 root, one Newton step with an FP32
+// reciprocal; relative error ~1e-14.
+__device__ __forceinline__ double sqrt64(double x)
+{
+    float sf = f_sqrt((float)x);
+    if (!(sf > 0.0f)) return 0.0;
+    double s = (double)sf;
+    double r = (double)(0.5f * f_rcp(sf));
+    return s + (x - s * s) * r;
+}
+
+// 1 / |d| in FP64 for an FP32-normalised d (|d|^2 - 1 ~ 1e-7): third-order
+// series of (1 + e)^(-1/2), exact to ~1e-20 there.
+__device__ __forceinline__ double inv_norm64(double dx, double dy, double dz)
+{
+    double e = dx * dx + dy * dy + dz * dz - 1.0;
+    return 1.0 + e * (-0.5 + e * (0.375 - 0.3125 * e));
+}
+
+// Quotient n / d for n < 2^24 from the float reciprocal rd = 1 / d (correctly
+// rounded): the float quotient is within 2 of the true one.
+__device__ __forceinline__ uint32_t udiv24(uint32_t n, uint32_t d, float rd)
+{
+    uint32_t q = (uint32_t)((float)n * rd);
+    int32_t r = (int32_t)(n - q * d);
+    if (r < 0) {
+        q--;
+        r += (int32_t)d;
+        if (r < 0) {
+            q--;
+            r += (int32_t)d;
+        }
+    }
+    if (r >= (int32_t)d) {
+        q++;
+        r -= (int32_t)d;
+        if (r >= (int32_t)d) q++;
+    }
+    return q;
+}
+
 // ---------------------------------------------------------------- RNG
 // pcg4d counter hash keyed (pixel ^ seed_lo, sample ^ seed_hi, band, dim)
 // (sky_trace.wgsl:156-189).  The first LCG step of the three per-path words
@@ -63,14 +132,17 @@ struct Rng {
     uint32_t w;        // dim * 1664525 + 1013904223
 };
 
+constexpr uint32_t kLcgMul = 1664525u;
+constexpr uint32_t kLcgAdd = 1013904223u;
+
 __device__ __forceinline__ Rng rng_make(uint32_t pixel, uint32_t sample, uint32_t band,
                                         const DevScene &s)
 {
     Rng r;
-    r.x = (pixel ^ s.seed_lo) * 1664525u + 1013904223u;
-    r.y = (sample ^ s.seed_hi) * 1664525u + 1013904223u;
-    r.z = band * 1664525u + 1013904223u;
-    r.w = 1013904223u;  // dim 0
+    r.x = (pixel ^ s.seed_lo) * kLcgMul + kLcgAdd;
+    r.y = (sample ^ s.seed_hi) * kLcgMul + kLcgAdd;
+    r.z = band * kLcgMul + kLcgAdd;
+    r.w = kLcgAdd;  // dim 0
     return r;
 }
 
@@ -91,18 +163,19 @@ __device__ __forceinline__ uint32_t pcg4d_fold(uint32_t x, uint32_t y, uint32_t 
     return x ^ y ^ z ^ w;
 }
 
+__device__ __forceinline__ float rng_unit(uint32_t v) { return (float)(v >> 8) * 5.9604644775390625e-8f; }
+
 __device__ __forceinline__ float rng_next(Rng &r)
 {
     uint32_t v = pcg4d_fold(r.x, r.y, r.z, r.w);
-    r.w += 1664525u;
-    return (float)(v >> 8) * 5.9604644775390625e-8f;
+    r.w += kLcgMul;
+    return rng_unit(v);
 }
 
 // Value of draw `ahead` positions after the current one, without advancing.
 __device__ __forceinline__ float rng_peek(const Rng &r, uint32_t ahead)
 {
-    uint32_t v = pcg4d_fold(r.x, r.y, r.z, r.w + ahead * 1664525u);
-    return (float)(v >> 8) * 5.9604644775390625e-8f;
+    return rng_unit(pcg4d_fold(r.x, r.y, r.z, r.w + ahead * kLcgMul));
 }
 
 // ------------------------------------------------------------ vectors
@@ -115,25 +188,28 @@ __device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y
 // normalize_or_y, sky_trace.wgsl:191-197.
 __device__ __forceinline__ f3 normalize_or_y(f3 v)
 {
-    float len = sqrtf(dot3(v, v));
-    if (len > 0.0f) return f3{v.x / len, v.y / len, v.z / len};
+    float l2 = dot3(v, v);
+    if (l2 > 0.0f) {
+        float r = f_rsq(l2);
+        return f3{v.x * r, v.y * r, v.z * r};
+    }
     return f3{0.0f, 1.0f, 0.0f};
 }
 
 // direction_from_axis_mu with the branchless orthonormal basis
-// (sky_trace.wgsl:512-532): polar cosine mu, polar sine st, azimuth phi.
-__device__ __forceinline__ f3 frame_dir(f3 axis, float mu, float st, float phi)
+// (sky_trace.wgsl:512-532): unit axis n, polar cosine mu, polar sine st,
+// azimuth phi_rev in revolutions.  The basis vectors of a unit axis are unit
+// vectors; only the result is renormalised.
+__device__ __forceinline__ f3 frame_dir(f3 n, float mu, float st, float phi_rev)
 {
-    f3 n = normalize_or_y(axis);
     float sign = n.z >= 0.0f ? 1.0f : -1.0f;
-    float a = -1.0f / (sign + n.z);
+    float a = -f_rcp(sign + n.z);
     float b = n.x * n.y * a;
-    f3 t = normalize_or_y(f3{1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x});
-    f3 bt = normalize_or_y(f3{b, sign + n.y * n.y * a, -n.y});
-    float sp, cp;
-    sincosf(phi, &sp, &cp);
-    f3 v{t.x * (st * cp) + bt.x * (st * sp) + n.x * mu, t.y * (st * cp) + bt.y * (st * sp) + n.y * mu,
-         t.z * (st * cp) + bt.z * (st * sp) + n.z * mu};
+    f3 t{1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x};
+    f3 bt{b, sign + n.y * n.y * a, -n.y};
+    float sp = f_sin_rev(phi_rev), cp = f_cos_rev(phi_rev);
+    float c1 = st * cp, c2 = st * sp;
+    f3 v{t.x * c1 + bt.x * c2 + n.x * mu, t.y * c1 + bt.y * c2 + n.y * mu, t.z * c1 + bt.z * c2 + n.z * mu};
     return normalize_or_y(v);
 }
 
@@ -151,21 +227,18 @@ struct Ray {
 
 // Ray set-up: next_boundary (sky_trace.wgsl:233-250) from the impact
 // parameter -- b^2 - c of wgsl:212-215 is D - q with D = r_shell^2 - R^2 --
-// and the initial shell of the walk.
+// and the initial shell of the walk.  d is an FP32-normalised direction.
 __device__ __forceinline__ void ray_setup(const DevScene &s, double ox, double oy, double oz, f3 d,
                                           Ray &r)
 {
     double dx = d.x, dy = d.y, dz = d.z;
-    double inv = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
-    dx *= inv;
-    dy *= inv;
-    dz *= inv;
+    double inv = inv_norm64(dx, dy, dz);
     r.ox = ox;
     r.oy = oy;
     r.oz = oz;
     r.d = d;
     double oo = ox * ox + oy * oy + oz * oz;
-    double b = ox * dx + oy * dy + oz * dz;
+    double b = (ox * dx + oy * dy + oz * dz) * inv;
     double q = oo - b * b - s.R * s.R;
     r.b = b;
     r.q = q;
@@ -174,10 +247,10 @@ __device__ __forceinline__ void ray_setup(const DevScene &s, double ox, double o
     double disc_top = s.delta_top - q;
     r.hit = disc_top >= 0.0;
     const double eps = (double)kRayEpsilonKm;
-    double t_exit = -b + sqrt(disc_top > 0.0 ? disc_top : 0.0);
+    double t_exit = -b + sqrt64(disc_top > 0.0 ? disc_top : 0.0);
     double tg = -1.0;
     if (-q >= 0.0) {
-        double sg = sqrt(-q);
+        double sg = sqrt64(-q);
         double t0 = -b - sg, t1 = -b + sg;
         tg = t0 > eps ? t0 : (t1 > eps ? t1 : -1.0);
     }
@@ -189,8 +262,8 @@ __device__ __forceinline__ void ray_setup(const DevScene &s, double ox, double o
         r.tmax = (float)t_exit;
         r.kind = 1;
     }
-    double h0 = sqrt(oo) - s.R;
-    int k = (int)floor(h0 / s.dz);
+    double h0 = sqrt64(oo) - s.R;
+    int k = (int)floor(h0 * s.inv_dz);
     r.outward = b >= 0.0;
     if (!r.outward && k > 0 && h0 - k * s.dz <= 1e-9) k -= 1;
     k = k < 0 ? 0 : k;
@@ -198,14 +271,20 @@ __device__ __forceinline__ void ray_setup(const DevScene &s, double ox, double o
     r.layer = k;
 }
 
+// r_k^2 - R^2 - q for shell k.
+__device__ __forceinline__ double shell_disc(const DevScene &s, int k, double q)
+{
+    double fk = (double)k;
+    return fk * (s.kd1 + fk * s.kd2) - q;
+}
+
 // Next crossing of the majorant-shell walk (the segments of
 // sky_trace.wgsl:284-294): D_k - q in FP64, its root in FP32.
-__device__ __forceinline__ float next_crossing(const DevScene &s, const Ray &r, int k, int out,
+__device__ __forceinline__ float next_crossing(const DevScene &s, double q, float bf, int k, int out,
                                                int &kn, int &on)
 {
     if (!out) {
-        double hk = k * s.dz;
-        double disc = hk * (2.0 * s.R + hk) - r.q;
+        double disc = shell_disc(s, k, q);
         if (disc > 0.0) {
             on = 0;
             if (k == 0) {
@@ -213,7 +292,7 @@ __device__ __forceinline__ float next_crossing(const DevScene &s, const Ray &r, 
                 return __builtin_inff();
             }
             kn = k - 1;
-            return -sqrtf((float)disc) - r.bf;
+            return -f_sqrt((float)disc) - bf;
         }
     }
     on = 1;
@@ -222,16 +301,27 @@ __device__ __forceinline__ float next_crossing(const DevScene &s, const Ray &r, 
         return __builtin_inff();
     }
     kn = k + 1;
-    double hk = (k + 1) * s.dz;
-    double disc = hk * (2.0 * s.R + hk) - r.q;
-    return sqrtf((float)(disc > 0.0 ? disc : 0.0)) - r.bf;
+    double disc = shell_disc(s, k + 1, q);
+    return f_sqrt((float)(disc > 0.0 ? disc : 0.0)) - bf;
+}
+
+__device__ __forceinline__ float next_crossing(const DevScene &s, const Ray &r, int k, int out, int &kn,
+                                               int &on)
+{
+    return next_crossing(s, r.q, r.bf, k, out, kn, on);
+}
+
+// Altitude at distance t: (q + (b + t)^2) / (sqrt(R^2 + q + (b + t)^2) + R).
+__device__ __forceinline__ float altitude_bq(const DevScene &s, float bf, float qf, float t)
+{
+    float sv = bf + t;
+    float w = qf + sv * sv;
+    return f_div(w, f_sqrt(s.R2f + w) + s.Rf);
 }
 
 __device__ __forceinline__ float altitude(const DevScene &s, const Ray &r, float t)
 {
-    float sv = r.bf + t;
-    float w = r.qf + sv * sv;
-    return w / (sqrtf(s.R2f + w) + s.Rf);
+    return altitude_bq(s, r.bf, r.qf, t);
 }
 
 // coefficients_at (sky_trace.wgsl:356-386) for the analytic profiles.
@@ -239,19 +329,24 @@ struct Coeffs {
     float sr, sas, so, saa;
 };
 
-__device__ __forceinline__ Coeffs medium(const DevScene &s, const DevLambda &l, float h)
+__device__ __forceinline__ Coeffs medium4(const DevScene &s, float br, float bo, float bas, float baa, float h)
 {
     Coeffs c{0.0f, 0.0f, 0.0f, 0.0f};
     if (h < 0.0f || h > s.topf) return c;
-    float rr = expf(-h * s.inv_hr);
-    float ra = expf(-h * s.inv_ha);
+    float rr = f_exp2(-h * s.l2_hr);
+    float ra = f_exp2(-h * s.l2_ha);
     float ro = 1.0f - fabsf(h - s.hc) * s.inv_hw;
     ro = ro < 0.0f ? 0.0f : ro;
-    c.sr = l.br * rr;
-    c.sas = l.bas * ra;
-    c.so = l.bo * ro;
-    c.saa = l.baa * ra;
+    c.sr = br * rr;
+    c.sas = bas * ra;
+    c.so = bo * ro;
+    c.saa = baa * ra;
     return c;
+}
+
+__device__ __forceinline__ Coeffs medium(const DevScene &s, const DevLambda &l, float h)
+{
+    return medium4(s, l.br, l.bo, l.bas, l.baa, h);
 }
 
 // ------------------------------------------------------------ phase
@@ -263,7 +358,7 @@ __device__ __forceinline__ float phase_rayleigh(float mu)
 __device__ __forceinline__ float phase_hg(float g, float mu)
 {
     float den = 1.0f + g * g - 2.0f * g * mu;
-    return (1.0f - g * g) / (4.0f * kPi * den * sqrtf(den));
+    return (1.0f - g * g) * (0.25f * kInvPi) * f_rsq(den) * f_rcp(den);
 }
 
 // mixed_phase_value, sky_trace.wgsl:496-510 (Rayleigh + HG aerosol).
@@ -271,34 +366,45 @@ __device__ __forceinline__ float mixed_phase(const DevScene &s, float sr, float 
 {
     float scat = sr + sas;
     if (scat <= 0.0f) return 0.0f;
-    return sr / scat * phase_rayleigh(mu) + sas / scat * phase_hg(s.g, mu);
+    return f_div(sr * phase_rayleigh(mu) + sas * phase_hg(s.g, mu), scat);
 }
 
-// invert_rayleigh_cdf, crates/sky-core/src/sampling.rs:182-185.
+// invert_rayleigh_cdf (crates/sky-core/src/sampling.rs:182-185): the root of
+// mu^3 + 3 mu = 2 y, y = 4 xi - 2, by Cardano: mu = A - 1/A with
+// A = cbrt(|y| + sqrt(y^2 + 1)) (odd in y; A in [1, 1.62]).
 __device__ __forceinline__ float sample_rayleigh_mu(float xi)
 {
     float y = 4.0f * xi - 2.0f;
-    float mu = 2.0f * sinhf(asinhf(y) / 3.0f);
+    float ay = fabsf(y);
+    float A = f_exp2(f_log2(ay + f_sqrt(fmaf(ay, ay, 1.0f))) * (1.0f / 3.0f));
+    float mu = A - f_rcp(A);
+    mu = y < 0.0f ? -mu : mu;
     return fminf(fmaxf(mu, -1.0f), 1.0f);
 }
 
 __device__ __forceinline__ float sample_hg_mu(float g, float xi)
 {
     if (fabsf(g) < 1e-3f) return 1.0f - 2.0f * xi;
-    float sv = (1.0f - g * g) / (1.0f - g + 2.0f * g * xi);
-    float mu = (1.0f + g * g - sv * sv) / (2.0f * g);
+    float sv = f_div(1.0f - g * g, 1.0f - g + 2.0f * g * xi);
+    float mu = f_div(1.0f + g * g - sv * sv, 2.0f * g);
     return fminf(fmaxf(mu, -1.0f), 1.0f);
 }
 
-// Sun cone sample (sky_trace.wgsl:541-546), 1 - cos(theta) formed from
-// 1 - cos(r) = 2 sin^2(r/2) so the 4.7 mrad cone keeps its solid angle.
+// Sun cone sample (sky_trace.wgsl:541-546) as (mu, st): 1 - cos(theta) is
+// formed from 1 - cos(r) = 2 sin^2(r/2) so the 4.7 mrad cone keeps its
+// solid angle.
+__device__ __forceinline__ void sun_cone_mu(const DevScene &s, float xi0, float &mu, float &st)
+{
+    float x = fminf(xi0, 0.99999994f) * s.omc;
+    mu = 1.0f - x;
+    st = f_sqrt(x * (2.0f - x));
+}
+
 __device__ __forceinline__ f3 sun_cone_dir(const DevScene &s, float xi0, float xi1)
 {
-    float xi = fminf(xi0, 0.99999994f);
-    float x = xi * s.omc;
-    float mu = 1.0f - x;
-    float st = sqrtf(x * (2.0f - x));
-    return frame_dir(f3{s.sun[0], s.sun[1], s.sun[2]}, mu, st, kTau * xi1);
+    float mu, st;
+    sun_cone_mu(s, xi0, mu, st);
+    return frame_dir(f3{s.sun[0], s.sun[1], s.sun[2]}, mu, st, xi1);
 }
 
 // direction_in_cone (sky_trace.wgsl:562-564) as |d - s|^2 <= 2 (1 - cos r).
@@ -314,39 +420,39 @@ __device__ __forceinline__ f3 camera_dir(const DevScene &s, uint32_t x, uint32_t
                                          float j2)
 {
     if (s.projection == 1u) {
-        float u = ((float)x + 0.5f) / (float)s.width;
-        float v = ((float)y + 0.5f) / (float)s.height;
-        float uu = s.width <= 1 ? 0.5f : (u - 0.5f / (float)s.width) * ((float)s.width / ((float)s.width - 1.0f));
-        float vv = s.height <= 1 ? 0.5f : (v - 0.5f / (float)s.height) * ((float)s.height / ((float)s.height - 1.0f));
+        float u = ((float)x + 0.5f) * s.inv_w;
+        float v = ((float)y + 0.5f) * s.inv_h;
+        float uu = s.width <= 1 ? 0.5f : f_div(u - 0.5f * s.inv_w, 1.0f - s.inv_w);
+        float vv = s.height <= 1 ? 0.5f : f_div(v - 0.5f * s.inv_h, 1.0f - s.inv_h);
         uu = fminf(fmaxf(uu, 0.0f), 1.0f);
         vv = fminf(fmaxf(vv, 0.0f), 1.0f);
         float vzc;
         if (vv < 0.75f) {
-            float c = vv / 0.75f;
+            float c = vv * (1.0f / 0.75f);
             c = 1.0f - c;
             c = c * c;
             c = 1.0f - c;
-            vzc = cosf(s.svl_zha * c);
+            vzc = f_cos_rev(s.svl_zha * c * kInv2Pi);
         } else {
-            float c = (vv - 0.75f) / 0.25f;
+            float c = (vv - 0.75f) * 4.0f;
             c = c * c;
-            vzc = cosf(s.svl_zha + s.svl_beta * c);
+            vzc = f_cos_rev((s.svl_zha + s.svl_beta * c) * kInv2Pi);
         }
         float c = uu * uu;
         float lvc = -(c * 2.0f - 1.0f);
-        float sv = sqrtf(fmaxf(1.0f - vzc * vzc, 0.0f));
-        float ss = sqrtf(fmaxf(1.0f - lvc * lvc, 0.0f));
+        float sv = f_sqrt(fmaxf(1.0f - vzc * vzc, 0.0f));
+        float ss = f_sqrt(fmaxf(1.0f - lvc * lvc, 0.0f));
         f3 v3{sv * (s.svl_fw[0] * lvc + s.svl_side[0] * ss), vzc + sv * (s.svl_fw[1] * lvc + s.svl_side[1] * ss),
               sv * (s.svl_fw[2] * lvc + s.svl_side[2] * ss)};
         return normalize_or_y(v3);
     }
-    float u = ((float)x + j1) / (float)s.width;
-    float v = ((float)y + j2) / (float)s.height;
-    float az = (u - 0.5f) * kTau;
-    float el = (0.5f - v) * kPi;
-    float se, ce, sa, ca;
-    sincosf(el, &se, &ce);
-    sincosf(az, &sa, &ca);
+    // azimuth (u - 1/2) 2 pi and elevation (1/2 - v) pi, in revolutions
+    float u = ((float)x + j1) * s.inv_w;
+    float v = ((float)y + j2) * s.inv_h;
+    float az = u - 0.5f;
+    float el = (0.5f - v) * 0.5f;
+    float se = f_sin_rev(el), ce = f_cos_rev(el);
+    float sa = f_sin_rev(az), ca = f_cos_rev(az);
     return normalize_or_y(f3{ce * sa, se, ce * ca});
 }
 

@@ -57,6 +57,7 @@ struct PoolArgs {
     uint32_t *counter;
     unsigned long long *t_start;
     uint32_t event_threshold, track_steps;
+    uint32_t maj_lds;  // floats per wave of the LDS majorant copy (0: read the grids from global memory)
 };
 
 // Inputs of the adjoint replay (sky_render_adjoint).
@@ -71,12 +72,15 @@ struct AdjArgs {
 // Launch-wide settings of the wave-pool integrator (sky_wavepool.hip).
 struct WavePlan {
     uint32_t n_items = 0, grid = 0, block = 256, event_threshold = 32, track_steps = 8, occupancy = 0;
+    uint32_t maj_lds = 0;   // floats per wave for the majorant grid in LDS (0: global)
+    uint32_t lds_bytes = 0; // dynamic LDS per block
 };
 
 void plan_job(const DevScene &s, const sky_tuning &t, int num_cus, uint32_t min_items, DevJob &j);
 size_t job_accum_bytes(const DevJob &j);
 uint32_t launch_min_items(const sky_tuning &t, int num_cus);
-WavePlan plan_launch(const sky_tuning &t, int num_cus, uint32_t total_items);
+// max_maj: the largest nlam x majorant_layers over the launch's jobs.
+WavePlan plan_launch(const sky_tuning &t, int num_cus, uint32_t total_items, uint32_t max_maj);
 hipError_t launch_pool(const DevJob *d_jobs, uint32_t n_jobs, uint32_t max_npx, const WavePlan &p,
                        uint32_t *meta, uint32_t meta_words, hipStream_t stream);
 hipError_t launch_zero(uint32_t *p, uint32_t n, hipStream_t stream);

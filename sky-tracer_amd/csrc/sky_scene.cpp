@@ -209,6 +209,9 @@ sky_status build_scene(const sky_planet *planet, const sky_atmosphere *a, const 
     s.top = planet->atmosphere_radius_km - planet->ground_radius_km;
     s.L = cfg->majorant_layers;
     s.dz = s.top / s.L;
+    s.inv_dz = 1.0 / s.dz;
+    s.kd1 = 2.0 * s.R * s.dz;
+    s.kd2 = s.dz * s.dz;
     s.delta_top = (planet->atmosphere_radius_km - s.R) * (planet->atmosphere_radius_km + s.R);
     s.obs_y = s.R + std::fmax(cfg->observer_altitude_km, 0.0);
     s.Rf = (float)s.R;
@@ -226,6 +229,8 @@ sky_status build_scene(const sky_planet *planet, const sky_atmosphere *a, const 
     s.cone_pdf = (float)(1.0 / (2.0 * kPiD * omc));
     s.inv_hr = (float)(1.0 / a->rayleigh_h_km);
     s.inv_ha = (float)(1.0 / a->aerosol_h_km);
+    s.l2_hr = (float)(1.4426950408889634 / a->rayleigh_h_km);
+    s.l2_ha = (float)(1.4426950408889634 / a->aerosol_h_km);
     s.hc = (float)a->ozone_center_km;
     s.inv_hw = (float)(1.0 / a->ozone_half_width_km);
     s.g = (float)a->aerosol_g;
@@ -254,6 +259,8 @@ sky_status build_scene(const sky_planet *planet, const sky_atmosphere *a, const 
     }
     s.width = cfg->width;
     s.height = cfg->height;
+    s.inv_w = (float)(1.0 / cfg->width);
+    s.inv_h = (float)(1.0 / cfg->height);
     s.spp = cfg->spp;
     s.sample_offset = cfg->sample_offset;
     s.seed_lo = (uint32_t)cfg->seed;

@@ -43,26 +43,22 @@ __device__ __forceinline__ bool track(const DevScene &s, const DevLambda &l, con
         float maj = maj_row[k];
         while (t < seg_end) {
             if (carried) {
-                if (tau < 0.0f) {
-                    float u = fmaxf(1.0f - rng_next(rng), 0.0000001f);
-                    tau = -logf(u);
-                }
+                if (tau < 0.0f) tau = free_flight(rng_next(rng));
                 float seg_tau = maj * (seg_end - t);
                 if (tau >= seg_tau) {
                     tau -= seg_tau;
                     break;
                 }
-                t = t + tau / maj;
+                t = t + f_div(tau, maj);
                 tau = -1.0f;
                 if (t >= seg_end) break;
             } else {
-                float u = fmaxf(1.0f - rng_next(rng), 0.0000001f);
-                t = t - logf(u) / maj;
+                t = t + f_div(free_flight(rng_next(rng)), maj);
                 if (t >= seg_end) break;
             }
             Coeffs c = medium(s, l, altitude(s, r, t));
             float st = c.sr + c.sas + c.so + c.saa;
-            float ratio = fminf(fmaxf(st / maj, 0.0f), 1.0f);
+            float ratio = fminf(fmaxf(f_div(st, maj), 0.0f), 1.0f);
             if (mode == kTrackRatio) {
                 T *= 1.0f - ratio;
                 continue;
@@ -133,7 +129,7 @@ __device__ float trace_path(const DevScene &s, uint32_t x, uint32_t y, uint32_t 
         if (track(s, l, maj_row, ray, rng, kTrackCollision, th, co, Tdummy, wd)) {
             // ray.d is FP32-normalised; positions use its FP64 unit vector.
             double dx = ray.d.x, dy = ray.d.y, dz = ray.d.z;
-            double inv = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+            double inv = inv_norm64(dx, dy, dz);
             double px = ray.ox + dx * inv * (double)th;
             double py = ray.oy + dy * inv * (double)th;
             double pz = ray.oz + dz * inv * (double)th;
@@ -164,17 +160,15 @@ __device__ float trace_path(const DevScene &s, uint32_t x, uint32_t y, uint32_t 
                 float xl = rng_next(rng);
                 float xm = rng_next(rng);
                 float mu = xl * scat < co.sr ? sample_rayleigh_mu(xm) : sample_hg_mu(s.g, xm);
-                float st = sqrtf(fmaxf(1.0f - mu * mu, 0.0f));
-                nd = frame_dir(ray.d, mu, st, kTau * rng_next(rng));
+                float st = f_sqrt(fmaxf(1.0f - mu * mu, 0.0f));
+                nd = frame_dir(ray.d, mu, st, rng_next(rng));
                 w = 1.0f;
             } else {
                 // sample_isotropic, wgsl:534-539
                 float z = 1.0f - 2.0f * rng_next(rng);
-                float rr = sqrtf(fmaxf(1.0f - z * z, 0.0f));
-                float phi = kTau * rng_next(rng);
-                float sp, cp;
-                sincosf(phi, &sp, &cp);
-                nd = f3{rr * cp, z, rr * sp};
+                float rr = f_sqrt(fmaxf(1.0f - z * z, 0.0f));
+                float phi = rng_next(rng);
+                nd = f3{rr * f_cos_rev(phi), z, rr * f_sin_rev(phi)};
                 w = mixed_phase(s, co.sr, co.sas, dot3(ray.d, nd)) / kIsotropicPdf;
             }
             tp = tp * w;
@@ -188,12 +182,11 @@ __device__ float trace_path(const DevScene &s, uint32_t x, uint32_t y, uint32_t 
             break;
         } else {
             double dx = ray.d.x, dy = ray.d.y, dz = ray.d.z;
-            double inv = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+            double inv = inv_norm64(dx, dy, dz);
             double px = ray.ox + dx * inv * (double)ray.tmax;
             double py = ray.oy + dy * inv * (double)ray.tmax;
             double pz = ray.oz + dz * inv * (double)ray.tmax;
-            double pn = 1.0 / sqrt(px * px + py * py + pz * pz);
-            f3 nrm{(float)(px * pn), (float)(py * pn), (float)(pz * pn)};
+            f3 nrm = normalize_or_y(f3{(float)px, (float)py, (float)pz});
             // ground_radiance (wgsl:587-608)
             float sum = 0.0f;
             for (uint32_t i = 0; i < n_light; ++i) {
@@ -214,9 +207,9 @@ __device__ float trace_path(const DevScene &s, uint32_t x, uint32_t y, uint32_t 
             L = L + tp * (sum / (float)n_light);
             // sample_cosine_hemisphere, wgsl:548-560
             float u = rng_next(rng);
-            float rr = sqrtf(u);
-            float phi = kTau * rng_next(rng);
-            float ct = sqrtf(fmaxf(1.0f - u, 0.0f));
+            float rr = f_sqrt(u);
+            float phi = rng_next(rng);
+            float ct = f_sqrt(fmaxf(1.0f - u, 0.0f));
             f3 nd = frame_dir(nrm, ct, rr, phi);
             float co_out = fmaxf(dot3(nrm, nd), 0.0f);
             float pdf = co_out * kInvPi;

@@ -8,19 +8,21 @@
 // path is keyed exactly as the reference keys it (pixel, sample, band, dim),
 // so which lane traces it changes nothing but the order of the float sums.
 //
-// A lane is a small state machine with a single tracking call site:
-//   EVENT : vertex/boundary/shadow result (or a new path) waiting for the
-//           event handler, which samples directions, sets up the next ray
-//           (FP64) and either returns the lane to TRACK or finishes the path;
-//   TRACK : one tracking micro-step per loop trip (at most one RNG draw):
-//           a majorant-shell crossing, a tentative collision, or its accept
-//           test (sample_real_collision / delta_transmittance,
-//           sky_trace.wgsl:405-475);
+// A lane is a small state machine:
+//   TRACK : one tracking step per loop trip: the shell crossings up to the
+//           next tentative collision (cheap, looped), then that collision
+//           and its accept test (sample_real_collision / delta_transmittance,
+//           sky_trace.wgsl:405-475), or the end of the ray;
+//   EVENT : a vertex/boundary/shadow result (or a new path) waiting for the
+//           event handler.  One handler pass resolves the event, draws the
+//           step's random numbers, samples ONE direction (sun cone, phase
+//           lobe, cosine lobe) and sets up ONE ray (FP64), so every lane in
+//           the pass runs the same instructions whatever its event was;
 //   NEED  : path finished, waits for the next triple;  IDLE : item drained.
 // The event handler runs when at least `event_threshold` lanes wait for it
-// (or nothing is tracking); up to `track_steps` micro-steps run between two
-// handler passes.  Per-(pixel, band) sums live in LDS (ds_add_f32) and are
-// written once per item to a [chunk][pixel][band] partial buffer that
+// (or nothing is tracking); up to `track_steps` tracking steps run between
+// two handler passes.  Per-(pixel, band) sums live in LDS (ds_add_f32) and
+// are written once per item to a [chunk][pixel][band] partial buffer that
 // k_finalize reduces in chunk order.
 #include "sky_internal.hpp"
 #include "sky_trace_common.hpp"
@@ -42,14 +44,26 @@ __device__ __forceinline__ uint32_t mbcnt(uint64_t m)
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
+// The job table is read-only during a launch.  Reading it through the
+// constant address space lets the compiler keep the wave-uniform scene fields
+// in scalar registers (scalar-cache loads) instead of re-fetching them with
+// vector loads inside divergent code.
+typedef const __attribute__((address_space(4))) DevJob *ConstJobPtr;
+
+__device__ __forceinline__ const DevJob &job_at(const DevJob *jobs, uint32_t j)
+{
+    ConstJobPtr cj = (ConstJobPtr)jobs;
+    return *(const DevJob *)(cj + j);
+}
+
 // Everything a lane keeps between loop trips.  The main ray starts at the
 // current vertex (px, py, pz): the camera position, or the last vertex moved
 // by the ray epsilon, exactly as the reference offsets new rays
 // (sky_trace.wgsl:744, 766).  During a shadow ray the vertex stays put.
 struct LaneCtx {
-    int st, ev, mode, vtype, kind, pend;
+    int st, ev, mode, vtype, kind;
     uint32_t slot, band;
-    float br, bo, bas, baa;  // this band's coefficients (medium evaluation)
+    float br, bo, bas, baa, lsun;  // this band's coefficients and sun radiance
     Rng rng;
     float tp, L;
     uint32_t depth;
@@ -62,7 +76,7 @@ struct LaneCtx {
     float bf, qf, tmax;
     double q;
     float t, tau, seg_end, maj;
-    float aux;  // accept ratio pending (delta) / transmittance weight (ratio, shadow result)
+    float aux;  // shadow-ray transmittance (0/1 delta, product for ratio tracking)
     int k, out, kn, on;
     uint32_t guard;
 };
@@ -102,8 +116,8 @@ __device__ __forceinline__ DSig dsig(const DevScene &s, const AdjArgs &A, uint32
     DSig d{};
     if (h < 0.0f || h > s.topf) return d;
     const float4 u = A.lam_units[band];
-    float rr = expf(-h * s.inv_hr);
-    float ra = expf(-h * s.inv_ha);
+    float rr = f_exp2(-h * s.l2_hr);
+    float ra = f_exp2(-h * s.l2_ha);
     float dh = h - s.hc;
     float tent = fmaxf(1.0f - fabsf(dh) * s.inv_hw, 0.0f);
     d.r0 = u.x * rr;
@@ -166,29 +180,38 @@ __device__ __forceinline__ void add_dlog_phase(const DevScene &s, const AdjArgs 
 }
 
 // Majorant of (band, shell): from the wave's LDS copy of the job's grid when
-// it fits (nlam * L <= kMajLds), else from global memory.  A crossing needs
-// it immediately, so the LDS copy removes a global-load latency from every
-// shell crossing.
-constexpr int kMajLds = 1024;
+// the launch reserved room for it (nlam * L <= kMajLdsMax for every job),
+// else from global memory.  A crossing needs it immediately, so the LDS copy
+// removes a global-load latency from every shell crossing.
+constexpr uint32_t kMajLdsMax = 2048;
+// Per-wave LDS layout (floats): sums [256], sums of squares [256], adjoint
+// gradient [16], majorant grid [maj_lds].
+constexpr uint32_t kLdsFixed = 2 * kSlotsPerWave + 16;
 
+typedef __attribute__((address_space(3))) const float *LdsFloatPtr;
+typedef __attribute__((address_space(1))) const float *GlobalFloatPtr;
+
+// The two sources carry their address spaces so that each load is a plain
+// LDS or global load (a generic pointer would make every read a flat load,
+// which waits on both memory counters).
 struct MajSrc {
-    const float *lds;  // wave's LDS copy or null
+    LdsFloatPtr lds;      // wave's LDS copy (when use_lds)
+    GlobalFloatPtr glob;  // the job's grid in global memory
+    bool use_lds;
 };
 
-__device__ __forceinline__ float majorant(const DevScene &s, const MajSrc &m, uint32_t band, int k)
+__device__ __forceinline__ float majorant(const MajSrc &m, uint32_t L, uint32_t band, int k)
 {
-    uint32_t i = band * s.L + (uint32_t)k;
-    return m.lds ? m.lds[i] : s.maj[i];
+    uint32_t i = band * L + (uint32_t)k;
+    if (m.use_lds) return m.lds[i];
+    return m.glob[i];
 }
 
 __device__ __forceinline__ void seg_begin(const DevScene &s, const MajSrc &m, LaneCtx &c)
 {
-    Ray r;
-    r.q = c.q;
-    r.bf = c.bf;
-    float tc = next_crossing(s, r, c.k, c.out, c.kn, c.on);
+    float tc = next_crossing(s, c.q, c.bf, c.k, c.out, c.kn, c.on);
     c.seg_end = fmaxf(fminf(tc, c.tmax), c.t);
-    c.maj = majorant(s, m, c.band, c.k);
+    c.maj = majorant(m, s.L, c.band, c.k);
 }
 
 // Sets up a ray from (ox, oy, oz) along d (FP64 set-up, sky_device.hpp) and
@@ -208,7 +231,6 @@ __device__ __forceinline__ bool ray_begin(const DevScene &s, const MajSrc &m, La
     c.t = 0.0f;
     c.tau = -1.0f;
     c.aux = 1.0f;
-    c.pend = 0;
     c.guard = 1;
     c.k = r.layer;
     c.out = r.outward;
@@ -219,78 +241,37 @@ __device__ __forceinline__ bool ray_begin(const DevScene &s, const MajSrc &m, La
 
 __device__ __forceinline__ float lane_altitude(const DevScene &s, const LaneCtx &c, float t)
 {
-    float sv = c.bf + t;
-    float w = c.qf + sv * sv;
-    return w / (sqrtf(s.R2f + w) + s.Rf);
+    return altitude_bq(s, c.bf, c.qf, t);
 }
 
 __device__ __forceinline__ Coeffs lane_medium(const DevScene &s, const LaneCtx &c, float h)
 {
-    DevLambda l;
-    l.br = c.br;
-    l.bo = c.bo;
-    l.bas = c.bas;
-    l.baa = c.baa;
-    return medium(s, l, h);
+    return medium4(s, c.br, c.bo, c.bas, c.baa, h);
 }
 
-// One tracking micro-step: at most one RNG draw.  An empty segment (its
-// end clamped to t) is crossed without a draw, as the reference's inner
-// `while t < segment_end` loop does.
+// One tracking step.  The walk consumes shell crossings (no draw for an
+// empty segment, as the reference's inner `while t < segment_end` loop; with
+// carried optical depth, a crossing uses up part of the current draw) until
+// a tentative collision lies inside the current segment or the ray ends.
+// The collision is then evaluated and accepted or rejected in the same step.
 template <bool ADJ>
 __device__ __forceinline__ void track_step(const DevScene &s, const MajSrc &m, LaneCtx &c, AdjLane &a,
                                            const AdjArgs &A, bool &wd)
 {
     const bool carried = (s.flags & kFlagCarriedTau) != 0;
-    const bool empty = !(c.t < c.seg_end);
-    const bool need_draw = c.pend || (!empty && (!carried || c.tau < 0.0f));
-    float u = 0.0f;
-    if (need_draw) u = rng_next(c.rng);
-    if (c.pend) {
-        c.pend = 0;
-        if (u < c.aux) {
-            c.st = ST_EVENT;
-            if (c.mode == kTrackCollision) {
-                c.ev = EV_HIT;
-            } else {
-                c.aux = 0.0f;
-                c.ev = EV_SHADOW;
-            }
-        } else {
-            if constexpr (ADJ) {
-                // null collision: d log(1 - sigma_t / mu) = -d sigma_t / (mu - sigma_t)
-                Coeffs co = lane_medium(s, c, lane_altitude(s, c, c.t));
-                float st = co.sr + co.sas + co.so + co.saa;
-                float f = -1.0f / fmaxf(c.maj - st, 1e-30f);
-                DSig d = dsig(s, A, c.band, lane_altitude(s, c, c.t), c.bo, co);
-                add_dsigma_t(c.mode == kTrackCollision ? a.S : a.SS, d, f);
-            }
-            c.aux = 1.0f;
-        }
-        return;
-    }
-    bool cross = true;
-    float tn = c.t;
-    if (!empty) {
-        float ff = 0.0f;
-        if (need_draw) ff = -logf(fmaxf(1.0f - u, 0.0000001f));
-        if (carried) {
-            if (c.tau < 0.0f) c.tau = ff;
-            float seg_tau = c.maj * (c.seg_end - c.t);
-            cross = c.tau >= seg_tau;
-            if (cross) {
+    float tn = 0.0f;
+    for (;;) {
+        if (c.t < c.seg_end) {
+            if (c.tau < 0.0f) c.tau = free_flight(rng_next(c.rng));
+            const float seg_tau = c.maj * (c.seg_end - c.t);
+            if (carried && c.tau >= seg_tau) {
                 c.tau -= seg_tau;
             } else {
-                tn = c.t + c.tau / c.maj;
+                tn = c.t + f_div(c.tau, c.maj);
                 c.tau = -1.0f;
-                cross = tn >= c.seg_end;
+                if (tn < c.seg_end) break;
             }
-        } else {
-            tn = c.t + ff / c.maj;
-            cross = tn >= c.seg_end;
         }
-    }
-    if (cross) {
         c.t = c.seg_end;
         if (c.t >= c.tmax) {
             c.st = ST_EVENT;
@@ -309,22 +290,35 @@ __device__ __forceinline__ void track_step(const DevScene &s, const MajSrc &m, L
         c.k = c.kn;
         c.out = c.on;
         seg_begin(s, m, c);
-        return;
     }
     c.t = tn;
-    Coeffs co = lane_medium(s, c, lane_altitude(s, c, c.t));
+    const float h = lane_altitude(s, c, tn);
+    Coeffs co = lane_medium(s, c, h);
     float st = co.sr + co.sas + co.so + co.saa;
-    float ratio = fminf(fmaxf(st / c.maj, 0.0f), 1.0f);
+    float ratio = fminf(fmaxf(f_div(st, c.maj), 0.0f), 1.0f);
     if (c.mode == kTrackRatio) {
         c.aux *= 1.0f - ratio;
         if constexpr (ADJ) {
             float f = -1.0f / fmaxf(c.maj - st, 1e-30f);
-            DSig d = dsig(s, A, c.band, lane_altitude(s, c, c.t), c.bo, co);
-            add_dsigma_t(a.SS, d, f);
+            add_dsigma_t(a.SS, dsig(s, A, c.band, h, c.bo, co), f);
+        }
+        return;
+    }
+    float u = rng_next(c.rng);
+    if (u < ratio) {
+        c.st = ST_EVENT;
+        if (c.mode == kTrackCollision) {
+            c.ev = EV_HIT;
+        } else {
+            c.aux = 0.0f;
+            c.ev = EV_SHADOW;
         }
     } else {
-        c.pend = 1;
-        c.aux = ratio;
+        if constexpr (ADJ) {
+            // null collision: d log(1 - sigma_t / mu) = -d sigma_t / (mu - sigma_t)
+            float f = -1.0f / fmaxf(c.maj - st, 1e-30f);
+            add_dsigma_t(c.mode == kTrackCollision ? a.S : a.SS, dsig(s, A, c.band, h, c.bo, co), f);
+        }
     }
 }
 
@@ -332,52 +326,56 @@ __device__ __forceinline__ void track_step(const DevScene &s, const MajSrc &m, L
 __device__ __forceinline__ void advance_vertex(LaneCtx &c, float t)
 {
     double dx = c.d.x, dy = c.d.y, dz = c.d.z;
-    double inv = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
-    c.px += dx * inv * (double)t;
-    c.py += dy * inv * (double)t;
-    c.pz += dz * inv * (double)t;
+    double f = inv_norm64(dx, dy, dz) * (double)t;
+    c.px += dx * f;
+    c.py += dy * f;
+    c.pz += dz * f;
 }
 
-enum Act : int { ACT_NONE = 0, ACT_NEE = 1, ACT_CONT = 2, ACT_MAIN = 3, ACT_FINISH = 4, ACT_TRACKING = 5 };
+enum Next : int { NX_NEE = 1, NX_CONT = 2, NX_MAIN = 3, NX_FINISH = 4 };
 
-// Event handler: sky_trace.wgsl:703-776 between two tracking passes.
+// Event handler: one step of trace_path (sky_trace.wgsl:703-776) between two
+// tracking passes.  NEE = one sun sample of direct_sun_at_scatter /
+// ground_radiance (wgsl:566-608); CONT = the NEE average, the bounce
+// direction, its weight and the roulette (wgsl:691-701, 751-770); MAIN = a
+// main ray from the vertex (or the camera).  A shadow ray that cannot reach
+// the sun leaves the lane in EVENT with a zero result for the next pass.
 template <bool ADJ>
 __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m, LaneCtx &c, AdjLane &a,
                                              const AdjArgs &A, float *acc_sum, float *acc_sq, float *acc_grad,
                                              bool &wd)
 {
-    const uint32_t n_light = s.dls ? s.dls : 1u;
-    int act = ACT_NONE;
-    f3 mdir = c.vin;  // direction of the next main ray
+    const uint32_t n_light = s.dls;
+    const bool phase_is = (s.flags & kFlagPhaseIS) != 0;
+    int nx = NX_FINISH;
+
+    // 1. the event
     if (c.ev == EV_START) {
-        act = ACT_MAIN;
+        nx = NX_MAIN;
     } else if (c.ev == EV_BOUNDARY) {
         if (c.kind == 1) {
             // exit: the sun disk is seen only by the camera ray (wgsl:745-749)
             if (c.depth == 0 && in_sun_cone(s, c.d)) {
-                float C = c.tp * s.lam[c.band].lsun;
+                float C = c.tp * c.lsun;
                 c.L = c.L + C;
                 if constexpr (ADJ) {
                     for (int k = 0; k < kNP; ++k) a.G[k] += a.w * C * a.S[k];
                 }
             }
-            act = ACT_FINISH;
-        } else if (c.kind == 2) {
-            act = ACT_FINISH;  // tracking watchdog
-        } else {
+        } else if (c.kind == 0) {
             advance_vertex(c, c.tmax);
             c.vtype = 1;
             c.nee_i = 0;
             c.nee_sum = 0.0f;
-            act = ACT_NEE;
+            nx = NX_NEE;
         }
+        // kind 2: tracking watchdog, the path ends
     } else if (c.ev == EV_HIT) {
         const float hv = lane_altitude(s, c, c.t);
         Coeffs co = lane_medium(s, c, hv);
         advance_vertex(c, c.t);
         float scat = co.sr + co.sas;
         float ext = scat + co.so + co.saa;
-        act = ACT_FINISH;
         if constexpr (ADJ) {
             // accepted collision times the albedo weight: d log(sigma_s / mu)
             if (scat > 0.0f) add_dsigma_s(a.S, dsig(s, A, c.band, hv, c.bo, co), 1.0f / scat);
@@ -386,14 +384,14 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
             a.vh = hv;
         }
         if (ext > 0.0f && scat > 0.0f) {
-            c.tp = c.tp * fminf(fmaxf(scat / ext, 0.0f), 1.0f);
+            c.tp = c.tp * fminf(fmaxf(f_div(scat, ext), 0.0f), 1.0f);
             if (c.tp > 0.0f) {
                 c.vtype = 0;
                 c.vin = c.d;
-                c.rw = co.sr / scat;
+                c.rw = f_div(co.sr, scat);
                 c.nee_i = 0;
                 c.nee_sum = 0.0f;
-                act = ACT_NEE;
+                nx = NX_NEE;
             }
         }
     } else {  // EV_SHADOW
@@ -404,97 +402,122 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
                 for (int k = 0; k < kNP; ++k) a.G[k] += a.w * C * a.SS[k];
         }
         c.nee_i++;
-        act = c.nee_i < n_light ? ACT_NEE : ACT_CONT;
+        nx = c.nee_i < n_light ? NX_NEE : NX_CONT;
     }
 
-    // Sun NEE: direct_sun_at_scatter / ground_radiance (wgsl:566-608).
-    while (act == ACT_NEE) {
-        float a0 = rng_next(c.rng);
-        float a1 = rng_next(c.rng);
-        f3 ld = sun_cone_dir(s, a0, a1);
-        float factor;
-        const float lsun = s.lam[c.band].lsun;
+    // 2. this pass's random numbers, in the reference's order: sun cone (2);
+    //    bounce direction (3 with phase sampling, else 2), then the roulette
+    //    draw from rr_start_depth on.
+    const bool scatter_is = c.vtype == 0 && phase_is;
+    uint32_t ndraw = 0;
+    if (nx == NX_NEE) ndraw = 2;
+    if (nx == NX_CONT) ndraw = (scatter_is ? 3u : 2u) + (c.depth >= s.rr_start ? 1u : 0u);
+    float u0 = 0.0f, u1 = 0.0f, u2 = 0.0f, u3 = 0.0f;
+    if (ndraw > 0) u0 = rng_peek(c.rng, 0);
+    if (ndraw > 1) u1 = rng_peek(c.rng, 1);
+    if (ndraw > 2) u2 = rng_peek(c.rng, 2);
+    if (ndraw > 3) u3 = rng_peek(c.rng, 3);
+    c.rng.w += ndraw * kLcgMul;
+
+    // 3. one direction sample: sun cone (NEE), phase lobe or isotropic
+    //    (sample_isotropic, wgsl:534-539, as a frame around +y with the
+    //    azimuth negated), cosine lobe (sample_cosine_hemisphere, wgsl:548-560)
+    f3 dir = c.vin;
+    f3 nrm{0.0f, 1.0f, 0.0f};
+    if (c.vtype == 1) nrm = normalize_or_y(f3{(float)c.px, (float)c.py, (float)c.pz});
+    if (nx == NX_NEE || nx == NX_CONT) {
+        f3 axis;
+        float mu, st, phi;
+        if (nx == NX_NEE) {
+            sun_cone_mu(s, u0, mu, st);
+            axis = f3{s.sun[0], s.sun[1], s.sun[2]};
+            phi = u1;
+        } else if (c.vtype == 0) {
+            if (phase_is) {
+                mu = u0 < c.rw ? sample_rayleigh_mu(u1) : sample_hg_mu(s.g, u1);
+                axis = c.vin;
+                phi = u2;
+            } else {
+                mu = 1.0f - 2.0f * u0;
+                axis = f3{0.0f, 1.0f, 0.0f};
+                phi = -u1;
+            }
+            st = f_sqrt(fmaxf(1.0f - mu * mu, 0.0f));
+        } else {
+            mu = f_sqrt(fmaxf(1.0f - u0, 0.0f));
+            st = f_sqrt(u0);
+            axis = nrm;
+            phi = u1;
+        }
+        dir = frame_dir(axis, mu, st, phi);
+    }
+
+    // 4. weights
+    bool need_ray = false;
+    int mode = kTrackCollision;
+    double ox = c.px, oy = c.py, oz = c.pz;
+    float factor = 0.0f;
+    if (nx == NX_NEE) {
         if (c.vtype == 1) {
-            double pn = 1.0 / sqrt(c.px * c.px + c.py * c.py + c.pz * c.pz);
-            float cs = fmaxf((float)(c.px * pn) * ld.x + (float)(c.py * pn) * ld.y + (float)(c.pz * pn) * ld.z, 0.0f);
-            factor = s.albedo * kInvPi * lsun * cs / s.cone_pdf;
+            float cs = fmaxf(dot3(nrm, dir), 0.0f);
+            factor = s.albedo * kInvPi * c.lsun * cs / s.cone_pdf;
+            need_ray = cs > 0.0f;
             if constexpr (ADJ) {
                 for (int k = 0; k < kNP; ++k) a.SS[k] = a.S[k];
                 if (s.albedo > 0.0f) a.SS[9] += 1.0f / s.albedo;
             }
-            if (!(cs > 0.0f)) {
-                c.nee_i++;
-                act = c.nee_i < n_light ? ACT_NEE : ACT_CONT;
-                continue;
-            }
         } else {
-            float mu = dot3(c.vin, ld);
+            float mu = dot3(c.vin, dir);
             float ph = c.rw * phase_rayleigh(mu) + (1.0f - c.rw) * phase_hg(s.g, mu);
-            factor = lsun * ph / s.cone_pdf;
+            factor = c.lsun * ph / s.cone_pdf;
+            need_ray = true;
             if constexpr (ADJ) {
                 for (int k = 0; k < kNP; ++k) a.SS[k] = a.S[k];
                 add_dlog_phase(s, A, a, c.band, mu, c.bo, a.SS);
             }
         }
-        bool hit = ray_begin(s, m, c, c.px + (double)ld.x * kRayEpsilonKm, c.py + (double)ld.y * kRayEpsilonKm,
-                             c.pz + (double)ld.z * kRayEpsilonKm, ld,
-                             (s.flags & kFlagRatio) ? kTrackRatio : kTrackDelta);
-        if (!(hit && c.kind == 1)) {
+        mode = (s.flags & kFlagRatio) ? kTrackRatio : kTrackDelta;
+        ox += (double)dir.x * kRayEpsilonKm;
+        oy += (double)dir.y * kRayEpsilonKm;
+        oz += (double)dir.z * kRayEpsilonKm;
+        if (!need_ray) {
+            // ground facing away from the sun: a zero sample
             c.st = ST_EVENT;
-            c.nee_i++;
-            act = c.nee_i < n_light ? ACT_NEE : ACT_CONT;
-            continue;
+            c.ev = EV_SHADOW;
+            c.nee_factor = 0.0f;
+            c.aux = 0.0f;
         }
-        c.nee_factor = factor;
-        act = ACT_TRACKING;
-    }
-
-    if (act == ACT_CONT) {
+    } else if (nx == NX_CONT) {
         c.L = c.L + c.tp * (c.nee_sum / (float)n_light);
-        f3 nd;
         float w;
         bool ok = true;
         if (c.vtype == 0) {
-            const float rw = c.rw;
-            if (s.flags & kFlagPhaseIS) {
-                float xl = rng_next(c.rng);
-                float xm = rng_next(c.rng);
-                float mu = xl < rw ? sample_rayleigh_mu(xm) : sample_hg_mu(s.g, xm);
-                float st = sqrtf(fmaxf(1.0f - mu * mu, 0.0f));
-                nd = frame_dir(c.vin, mu, st, kTau * rng_next(c.rng));
-                w = 1.0f;
-                if constexpr (ADJ) add_dlog_phase(s, A, a, c.band, dot3(c.vin, nd), c.bo, a.S);
-            } else {
-                float z = 1.0f - 2.0f * rng_next(c.rng);
-                float rr = sqrtf(fmaxf(1.0f - z * z, 0.0f));
-                float phi = kTau * rng_next(c.rng);
-                float sp, cp;
-                sincosf(phi, &sp, &cp);
-                nd = f3{rr * cp, z, rr * sp};
-                float mu = dot3(c.vin, nd);
-                w = (rw * phase_rayleigh(mu) + (1.0f - rw) * phase_hg(s.g, mu)) / kIsotropicPdf;
-                if constexpr (ADJ) add_dlog_phase(s, A, a, c.band, mu, c.bo, a.S);
-            }
+            float mu = dot3(c.vin, dir);
+            w = phase_is ? 1.0f
+                         : (c.rw * phase_rayleigh(mu) + (1.0f - c.rw) * phase_hg(s.g, mu)) * (1.0f / kIsotropicPdf);
+            if constexpr (ADJ) add_dlog_phase(s, A, a, c.band, mu, c.bo, a.S);
         } else {
-            double pn = 1.0 / sqrt(c.px * c.px + c.py * c.py + c.pz * c.pz);
-            f3 nrm{(float)(c.px * pn), (float)(c.py * pn), (float)(c.pz * pn)};
-            float u = rng_next(c.rng);
-            float rr = sqrtf(u);
-            float phi = kTau * rng_next(c.rng);
-            float ct = sqrtf(fmaxf(1.0f - u, 0.0f));
-            nd = frame_dir(nrm, ct, rr, phi);
-            float co_out = fmaxf(dot3(nrm, nd), 0.0f);
-            float pdf = co_out * kInvPi;
-            ok = pdf > 0.0f && co_out > 0.0f;
-            w = ok ? s.albedo * kInvPi * co_out / pdf : 0.0f;
+            // albedo/pi cos / (cos/pi)
+            float co_out = fmaxf(dot3(nrm, dir), 0.0f);
+            ok = co_out > 0.0f;
+            w = s.albedo;
             if constexpr (ADJ) {
                 if (s.albedo > 0.0f) a.S[9] += 1.0f / s.albedo;
             }
         }
-        act = ACT_FINISH;
+        nx = NX_FINISH;
         if (ok) {
             c.tp = c.tp * w;
-            if (roulette(s, c.tp, c.depth, c.rng)) {
+            bool live = true;
+            if (c.depth >= s.rr_start) {
+                float q = fminf(fmaxf(c.tp, 0.05f), 0.95f);
+                float urr = scatter_is ? u3 : u2;
+                if (urr > q)
+                    live = false;
+                else
+                    c.tp = c.tp / q;
+            }
+            if (live) {
                 if (c.L < 0.0f || c.L != c.L || c.tp < 0.0f || c.tp != c.tp) {
                     c.L = 0.0f;
                     if constexpr (ADJ) {
@@ -502,29 +525,48 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
                     }
                 } else {
                     c.depth++;
-                    c.px += (double)nd.x * kRayEpsilonKm;
-                    c.py += (double)nd.y * kRayEpsilonKm;
-                    c.pz += (double)nd.z * kRayEpsilonKm;
-                    mdir = nd;
-                    act = ACT_MAIN;
+                    c.px += (double)dir.x * kRayEpsilonKm;
+                    c.py += (double)dir.y * kRayEpsilonKm;
+                    c.pz += (double)dir.z * kRayEpsilonKm;
+                    ox = c.px;
+                    oy = c.py;
+                    oz = c.pz;
+                    nx = NX_MAIN;
                 }
             }
         }
     }
-
-    if (act == ACT_MAIN) {
+    if (nx == NX_MAIN) {
         // loop top of trace_path (wgsl:709-717) plus the depth cap
-        act = ACT_FINISH;
+        nx = NX_FINISH;
         if (!(s.max_depth && c.depth >= s.max_depth)) {
             if (c.depth >= s.watchdog_limit) {
                 wd = true;
-            } else if (ray_begin(s, m, c, c.px, c.py, c.pz, mdir, kTrackCollision)) {
-                act = ACT_TRACKING;
+            } else {
+                need_ray = true;
+                mode = kTrackCollision;
+                nx = NX_MAIN;
             }
         }
     }
 
-    if (act == ACT_FINISH) {
+    // 5. one ray set-up
+    if (need_ray) {
+        bool hit = ray_begin(s, m, c, ox, oy, oz, dir, mode);
+        if (nx == NX_NEE) {
+            c.nee_factor = factor;
+            if (!(hit && c.kind == 1)) {
+                c.st = ST_EVENT;
+                c.ev = EV_SHADOW;
+                c.aux = 0.0f;
+            }
+        } else if (!hit) {
+            nx = NX_FINISH;
+        }
+    }
+
+    // 6. path end
+    if (nx == NX_FINISH) {
         float Lf = fmaxf(c.L, 0.0f);
         atomicAdd(&acc_sum[c.slot], Lf);
         atomicAdd(&acc_sq[c.slot], Lf * Lf);
@@ -536,27 +578,46 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
     }
 }
 
-// Path start (main, sky_trace.wgsl:799-807): triple j of the item.
+// Work-item constants shared by the lanes of a wave.
+struct ItemInfo {
+    uint32_t p0, x0, y0, nslots, s0;
+    float r_nslots, r_nlam, r_width;
+};
+
+// Path start (main, sky_trace.wgsl:799-807): triple j of the item, ordered
+// (sample, pixel, band) with the band fastest.
 template <bool ADJ>
 __device__ __forceinline__ void start_path(const DevScene &s, LaneCtx &c, AdjLane &a, const AdjArgs &A,
-                                           uint32_t j, uint32_t p0, uint32_t np, uint32_t s0)
+                                           uint32_t j, const ItemInfo &it)
 {
-    uint32_t per_sample = np * s.nlam;
-    uint32_t si = j / per_sample;
-    uint32_t rem = j - si * per_sample;
-    uint32_t pl = rem / s.nlam;
+    uint32_t si = udiv24(j, it.nslots, it.r_nslots);
+    uint32_t rem = j - si * it.nslots;
+    uint32_t pl = udiv24(rem, s.nlam, it.r_nlam);
     c.band = rem - pl * s.nlam;
     c.slot = rem;
-    uint32_t pixel = p0 + pl;
-    uint32_t x = pixel % s.width, y = pixel / s.width;
+    // pixel p0 + pl, from the item's first pixel (x0, y0); pl < 256
+    uint32_t x = it.x0 + pl, y = it.y0;
+    if (s.width >= 256u) {
+        if (x >= s.width) {
+            x -= s.width;
+            y++;
+        }
+    } else {
+        uint32_t dy = udiv24(x, s.width, it.r_width);
+        x -= dy * s.width;
+        y += dy;
+    }
+    const uint32_t pixel = it.p0 + pl;
     const DevLambda l = s.lam[c.band];
     c.br = l.br;
     c.bo = l.bo;
     c.bas = l.bas;
     c.baa = l.baa;
-    c.rng = rng_make(pixel, s0 + si, c.band, s);
-    float j1 = rng_next(c.rng);
-    float j2 = rng_next(c.rng);
+    c.lsun = l.lsun;
+    c.rng = rng_make(pixel, it.s0 + si, c.band, s);
+    float j1 = rng_peek(c.rng, 0);
+    float j2 = rng_peek(c.rng, 1);
+    c.rng.w += 2u * kLcgMul;
     c.vin = camera_dir(s, x, y, j1, j2);
     c.px = 0.0;
     c.py = s.obs_y;
@@ -564,6 +625,7 @@ __device__ __forceinline__ void start_path(const DevScene &s, LaneCtx &c, AdjLan
     c.tp = 1.0f;
     c.L = 0.0f;
     c.depth = 0;
+    c.vtype = 0;
     c.st = ST_EVENT;
     c.ev = EV_START;
     if constexpr (ADJ) {
@@ -583,27 +645,26 @@ __device__ __forceinline__ uint32_t find_job(const PoolArgs &a, uint32_t item)
     uint32_t lo = 0, hi = a.n_jobs;
     while (hi - lo > 1) {
         uint32_t mid = (lo + hi) >> 1;
-        if (a.jobs[mid].item_base <= item)
+        if (job_at(a.jobs, mid).item_base <= item)
             lo = mid;
         else
             hi = mid;
     }
-    return lo;
+    return __builtin_amdgcn_readfirstlane(lo);
 }
 
 template <bool ADJ>
 __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &A)
 {
-    __shared__ float lds_sum[kWavesPerBlock][kSlotsPerWave];
-    __shared__ float lds_sq[kWavesPerBlock][kSlotsPerWave];
-    __shared__ float lds_grad[kWavesPerBlock][ADJ ? 16 : 1];
-    __shared__ float lds_maj[kWavesPerBlock][kMajLds];
-    const DevJob *loaded = nullptr;  // job whose majorants are in lds_maj
+    extern __shared__ float dyn_lds[];
+    uint32_t loaded = 0xFFFFFFFFu;  // job whose majorants are in the wave's LDS copy
     const uint32_t wid = threadIdx.x >> 6;
     const uint32_t lane = lane_id();
-    float *acc_sum = lds_sum[wid];
-    float *acc_sq = lds_sq[wid];
-    float *acc_grad = lds_grad[wid];
+    float *wave_lds = dyn_lds + wid * (kLdsFixed + a.maj_lds);
+    float *acc_sum = wave_lds;
+    float *acc_sq = wave_lds + kSlotsPerWave;
+    float *acc_grad = wave_lds + 2 * kSlotsPerWave;
+    float *lds_maj = wave_lds + kLdsFixed;
     bool wd = false;
     uint32_t *diag = nullptr;
     LaneCtx c;
@@ -614,29 +675,38 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
         item = __builtin_amdgcn_readfirstlane(item);
         if (item >= a.total_items) break;
         if (item == 0 && lane == 0) *a.t_start = wall_clock64();
-        const DevJob &J = a.jobs[find_job(a, item)];
+        const uint32_t jid = find_job(a, item);
+        const DevJob &J = job_at(a.jobs, jid);
         const DevScene &s = J.s;
         diag = s.diag;
         const uint32_t npx = s.width * s.height;
         const uint32_t local = item - J.item_base;
         const uint32_t g = local % J.groups, ch = local / J.groups;
-        const uint32_t p0 = g * J.pixels_per_item;
-        const uint32_t np = min(J.pixels_per_item, npx - p0);
-        const uint32_t s0 = s.sample_offset + ch * J.samples_per_chunk;
+        ItemInfo it;
+        it.p0 = g * J.pixels_per_item;
+        const uint32_t np = min(J.pixels_per_item, npx - it.p0);
+        it.x0 = it.p0 % s.width;
+        it.y0 = it.p0 / s.width;
+        it.s0 = s.sample_offset + ch * J.samples_per_chunk;
         const uint32_t ns = min(J.samples_per_chunk, s.spp - ch * J.samples_per_chunk);
-        const uint32_t nslots = np * s.nlam;
-        const uint32_t total = nslots * ns;
-        for (uint32_t i = lane; i < nslots; i += 64) {
+        it.nslots = np * s.nlam;
+        it.r_nslots = 1.0f / (float)it.nslots;
+        it.r_nlam = 1.0f / (float)s.nlam;
+        it.r_width = 1.0f / (float)s.width;
+        const uint32_t total = it.nslots * ns;
+        for (uint32_t i = lane; i < it.nslots; i += 64) {
             acc_sum[i] = 0.0f;
             acc_sq[i] = 0.0f;
         }
-        MajSrc msrc{nullptr};
+        MajSrc msrc;
+        msrc.lds = (LdsFloatPtr)lds_maj;
+        msrc.glob = (GlobalFloatPtr)s.maj;
         const uint32_t nmaj = s.nlam * s.L;
-        if (nmaj <= (uint32_t)kMajLds) {
-            msrc.lds = lds_maj[wid];
-            if (loaded != &J) {
-                for (uint32_t i = lane; i < nmaj; i += 64) lds_maj[wid][i] = s.maj[i];
-                loaded = &J;
+        msrc.use_lds = nmaj <= a.maj_lds;
+        if (msrc.use_lds) {
+            if (loaded != jid) {
+                for (uint32_t i = lane; i < nmaj; i += 64) lds_maj[i] = s.maj[i];
+                loaded = jid;
             }
         }
         if constexpr (ADJ) {
@@ -652,7 +722,7 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
                 if (c.st == ST_NEED) {
                     uint32_t j = next + mbcnt(need);
                     if (j < total)
-                        start_path<ADJ>(s, c, al, A, j, p0, np, s0);
+                        start_path<ADJ>(s, c, al, A, j, it);
                     else
                         c.st = ST_IDLE;
                 }
@@ -671,8 +741,8 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        float2 *dst = J.accum + ((size_t)ch * npx + p0) * s.nlam;
-        for (uint32_t i = lane; i < nslots; i += 64) dst[i] = make_float2(acc_sum[i], acc_sq[i]);
+        float2 *dst = J.accum + ((size_t)ch * npx + it.p0) * s.nlam;
+        for (uint32_t i = lane; i < it.nslots; i += 64) dst[i] = make_float2(acc_sum[i], acc_sq[i]);
         if constexpr (ADJ) {
             if (lane < kNP) atomicAdd(&A.grad[lane], (double)acc_grad[lane]);
         }
@@ -686,15 +756,23 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
     if (wd && diag) atomicOr(diag, 1u);
 }
 
-// The same integrator under two register budgets: the compiler's own
-// allocation (3 waves/SIMD) and a 128-VGPR cap (4 waves/SIMD, with a few
-// spills in the event handler).  The autotuner picks one per workload.
+// The same integrator under several register budgets: the compiler's own
+// allocation and caps for at least 4, 5, 6 or 8 waves per SIMD (512 VGPRs per
+// lane / waves, 8-register granules: 128, 96, 80, 64).  The autotuner picks
+// one per workload.
 __global__ __launch_bounds__(256) void k_wavepool(PoolArgs a) { wavepool_body<false>(a, AdjArgs{}); }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_wavepool_o4(PoolArgs a)
-{
-    wavepool_body<false>(a, AdjArgs{});
-}
+#define SKY_WAVEPOOL_OCC(N)                                                                          \
+    __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N))) void k_wavepool_o##N( \
+        PoolArgs a)                                                                                  \
+    {                                                                                                \
+        wavepool_body<false>(a, AdjArgs{});                                                          \
+    }
+SKY_WAVEPOOL_OCC(4)
+SKY_WAVEPOOL_OCC(5)
+SKY_WAVEPOOL_OCC(6)
+SKY_WAVEPOOL_OCC(8)
+#undef SKY_WAVEPOOL_OCC
 
 // Path-replay adjoint: the forward state machine plus the score bookkeeping.
 __global__ __launch_bounds__(256) void k_wavepool_adjoint(PoolArgs a, AdjArgs A) { wavepool_body<true>(a, A); }
@@ -704,7 +782,7 @@ __global__ __launch_bounds__(256) void k_wavepool_adjoint(PoolArgs a, AdjArgs A)
 // XYZ = sum_band film * cmf * weight.
 __global__ __launch_bounds__(256) void k_finalize(const DevJob *jobs)
 {
-    const DevJob &J = jobs[blockIdx.y];
+    const DevJob &J = job_at(jobs, blockIdx.y);
     const DevScene &s = J.s;
     uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t npx = s.width * s.height;
@@ -741,6 +819,10 @@ __global__ void k_zero_u32(uint32_t *p, uint32_t n)
     if (i < n) p[i] = 0u;
 }
 
+// Triples of one item are numbered j < pixels x bands x samples, which the
+// kernel splits with float reciprocals: exact below 2^24.
+constexpr uint32_t kMaxItemTriples = 1u << 24;
+
 void plan_job(const DevScene &s, const sky_tuning &t, int num_cus, uint32_t min_items, DevJob &j)
 {
     const uint32_t npx = s.width * s.height;
@@ -753,6 +835,10 @@ void plan_job(const DevScene &s, const sky_tuning &t, int num_cus, uint32_t min_
     uint64_t max_chunks = s.spp / 16 ? s.spp / 16 : 1;
     if (chunks > max_chunks) chunks = max_chunks;
     if (chunks < 1) chunks = 1;
+    const uint64_t slots = (uint64_t)j.pixels_per_item * s.nlam;
+    const uint64_t max_spc = kMaxItemTriples / slots;
+    const uint64_t min_chunks = (s.spp + max_spc - 1) / max_spc;
+    if (chunks < min_chunks) chunks = min_chunks;
     (void)num_cus;
     j.samples_per_chunk = (uint32_t)((s.spp + chunks - 1) / chunks);
     j.chunks = (s.spp + j.samples_per_chunk - 1) / j.samples_per_chunk;
@@ -764,9 +850,11 @@ size_t job_accum_bytes(const DevJob &j)
     return (size_t)j.chunks * j.s.width * j.s.height * j.s.nlam * sizeof(float2);
 }
 
-WavePlan plan_launch(const sky_tuning &t, int num_cus, uint32_t total_items)
+WavePlan plan_launch(const sky_tuning &t, int num_cus, uint32_t total_items, uint32_t max_maj)
 {
     WavePlan p;
+    p.maj_lds = max_maj <= kMajLdsMax ? (max_maj + 63u) & ~63u : 0u;
+    p.lds_bytes = kWavesPerBlock * (kLdsFixed + p.maj_lds) * (uint32_t)sizeof(float);
     uint32_t wpc = t.waves_per_cu ? t.waves_per_cu : 16u;
     uint64_t waves = (uint64_t)(num_cus > 0 ? num_cus : 256) * wpc;
     uint64_t w = waves < total_items ? waves : total_items;
@@ -799,10 +887,14 @@ hipError_t launch_pool(const DevJob *d_jobs, uint32_t n_jobs, uint32_t max_npx, 
     a.t_start = (unsigned long long *)(meta + 2);
     a.event_threshold = p.event_threshold;
     a.track_steps = p.track_steps;
-    if (p.occupancy == 4)
-        hipLaunchKernelGGL(k_wavepool_o4, dim3(p.grid), dim3(p.block), 0, stream, a);
-    else
-        hipLaunchKernelGGL(k_wavepool, dim3(p.grid), dim3(p.block), 0, stream, a);
+    a.maj_lds = p.maj_lds;
+    switch (p.occupancy) {
+    case 4: hipLaunchKernelGGL(k_wavepool_o4, dim3(p.grid), dim3(p.block), p.lds_bytes, stream, a); break;
+    case 5: hipLaunchKernelGGL(k_wavepool_o5, dim3(p.grid), dim3(p.block), p.lds_bytes, stream, a); break;
+    case 6: hipLaunchKernelGGL(k_wavepool_o6, dim3(p.grid), dim3(p.block), p.lds_bytes, stream, a); break;
+    case 8: hipLaunchKernelGGL(k_wavepool_o8, dim3(p.grid), dim3(p.block), p.lds_bytes, stream, a); break;
+    default: hipLaunchKernelGGL(k_wavepool, dim3(p.grid), dim3(p.block), p.lds_bytes, stream, a); break;
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_finalize, dim3((max_npx + 255) / 256, n_jobs), dim3(256), 0, stream, d_jobs);
@@ -821,7 +913,8 @@ hipError_t launch_adjoint(const DevJob *d_jobs, const WavePlan &p, uint32_t *met
     a.t_start = (unsigned long long *)(meta + 2);
     a.event_threshold = p.event_threshold;
     a.track_steps = p.track_steps;
-    hipLaunchKernelGGL(k_wavepool_adjoint, dim3(p.grid), dim3(p.block), 0, stream, a, A);
+    a.maj_lds = p.maj_lds;
+    hipLaunchKernelGGL(k_wavepool_adjoint, dim3(p.grid), dim3(p.block), p.lds_bytes, stream, a, A);
     return hipGetLastError();
 }
 

@@ -27,7 +27,7 @@ SPACE = {
     "majorant_layers": [8, 12, 16, 24, 32, 64],
     "event_threshold": [8, 16, 24, 32, 40, 48, 56],
     "track_steps": [1, 2, 4, 8, 16, 32],
-    "occupancy": [0, 4],
+    "occupancy": [0, 4, 5, 6, 8],
     "waves_per_cu": [8, 12, 16, 24, 32],
     "pixels_per_item": [0, 1, 2, 4, 8],
 }

@@ -30,7 +30,12 @@ def test_paths_pair_with_oracle(gpu_context, mode, simple):
     film = _gpu(scene, cfg, gpu_context).values()
     ref, _, _, _, rc = O.render(op, "f32")
     assert rc == 0
-    frac, bad = paired_stats(film, ref.ravel(), rtol=1e-3)
+    # Ratio-tracked shadow rays return a continuous product of (1 - sigma_t /
+    # majorant) factors, so last-bit differences in sigma_t move a path's value
+    # without changing any decision: the FP32 and FP64 oracles themselves pair
+    # at only ~99.45% with rtol 1e-3 there (test_ratio_tracking_rounding_sensitivity).
+    rtol = 1e-2 if mode == "carried_is_ratio" else 1e-3
+    frac, bad = paired_stats(film, ref.ravel(), rtol=rtol)
     print(f"paired fraction {frac:.5f} ({len(bad)} of {film.size} differ)")
     assert frac >= 0.995
     assert np.all(np.isfinite(film)) and np.all(film >= 0)
