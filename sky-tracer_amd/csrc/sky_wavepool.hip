@@ -13,11 +13,14 @@
 //           next tentative collision (cheap, looped), then that collision
 //           and its accept test (sample_real_collision / delta_transmittance,
 //           sky_trace.wgsl:405-475), or the end of the ray;
-//   EVENT : a vertex/boundary/shadow result (or a new path) waiting for the
-//           event handler.  One handler pass resolves the event, draws the
-//           step's random numbers, samples ONE direction (sun cone, phase
-//           lobe, cosine lobe) and sets up ONE ray (FP64), so every lane in
-//           the pass runs the same instructions whatever its event was;
+//   EVENT : a vertex/boundary/shadow result waiting for the event handler.
+//           One handler pass resolves the event, draws the step's random
+//           numbers and samples ONE direction (sun cone, phase lobe, cosine
+//           lobe); lanes whose path ended pull their next triple (camera
+//           ray); then ONE ray set-up (FP64) serves every lane that needs a
+//           ray, so the lanes of a pass run the same instructions whatever
+//           their event was;
+//   RAY   : direction chosen, ray set-up pending (inside a handler pass);
 //   NEED  : path finished, waits for the next triple;  IDLE : item drained.
 // The event handler runs when at least `event_threshold` lanes wait for it
 // (or nothing is tracking); up to `track_steps` tracking steps run between
@@ -32,8 +35,8 @@ namespace sky {
 constexpr int kWavesPerBlock = 4;
 constexpr int kSlotsPerWave = 256;
 
-enum LaneState : int { ST_NEED = 0, ST_EVENT = 1, ST_TRACK = 2, ST_IDLE = 3 };
-enum LaneEvent : int { EV_START = 0, EV_HIT = 1, EV_BOUNDARY = 2, EV_SHADOW = 3 };
+enum LaneState : int { ST_NEED = 0, ST_EVENT = 1, ST_TRACK = 2, ST_IDLE = 3, ST_RAY = 4 };
+enum LaneEvent : int { EV_HIT = 1, EV_BOUNDARY = 2, EV_SHADOW = 3 };
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 
@@ -309,6 +312,22 @@ __device__ __forceinline__ void track_step(const DevScene &s, const MajSrc &m, L
         c.st = ST_EVENT;
         if (c.mode == kTrackCollision) {
             c.ev = EV_HIT;
+            // the vertex's albedo weight and Rayleigh share (wgsl:722-731);
+            // a zero throughput ends the path at the handler
+            float scat = co.sr + co.sas;
+            if constexpr (ADJ) {
+                // accepted collision times the albedo weight: d log(sigma_s / mu)
+                if (scat > 0.0f) add_dsigma_s(a.S, dsig(s, A, c.band, h, c.bo, co), 1.0f / scat);
+                a.vsr = co.sr;
+                a.vsas = co.sas;
+                a.vh = h;
+            }
+            if (st > 0.0f && scat > 0.0f) {
+                c.tp = c.tp * fminf(fmaxf(f_div(scat, st), 0.0f), 1.0f);
+                c.rw = f_div(co.sr, scat);
+            } else {
+                c.tp = 0.0f;
+            }
         } else {
             c.aux = 0.0f;
             c.ev = EV_SHADOW;
@@ -334,12 +353,29 @@ __device__ __forceinline__ void advance_vertex(LaneCtx &c, float t)
 
 enum Next : int { NX_NEE = 1, NX_CONT = 2, NX_MAIN = 3, NX_FINISH = 4 };
 
+// Path end: the path's radiance and its square into the wave's LDS sums.
+template <bool ADJ>
+__device__ __forceinline__ void finish_path(LaneCtx &c, AdjLane &a, float *acc_sum, float *acc_sq,
+                                            float *acc_grad)
+{
+    float Lf = fmaxf(c.L, 0.0f);
+    atomicAdd(&acc_sum[c.slot], Lf);
+    atomicAdd(&acc_sq[c.slot], Lf * Lf);
+    if constexpr (ADJ) {
+        for (int k = 0; k < kNP; ++k)
+            if (a.G[k] != 0.0f && a.G[k] == a.G[k]) atomicAdd(&acc_grad[k], a.G[k]);
+    }
+    c.st = ST_NEED;
+}
+
 // Event handler: one step of trace_path (sky_trace.wgsl:703-776) between two
 // tracking passes.  NEE = one sun sample of direct_sun_at_scatter /
 // ground_radiance (wgsl:566-608); CONT = the NEE average, the bounce
-// direction, its weight and the roulette (wgsl:691-701, 751-770); MAIN = a
-// main ray from the vertex (or the camera).  A shadow ray that cannot reach
-// the sun leaves the lane in EVENT with a zero result for the next pass.
+// direction, its weight and the roulette (wgsl:691-701, 751-770); MAIN = the
+// next main ray.  This is the first half of a handler pass, everything up to
+// the ray set-up: it leaves the lane in RAY (c.d, c.mode and, for a shadow
+// ray, c.nee_factor set), EVENT (a zero sun sample, for the next pass) or
+// NEED (path finished and accumulated).
 template <bool ADJ>
 __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m, LaneCtx &c, AdjLane &a,
                                              const AdjArgs &A, float *acc_sum, float *acc_sq, float *acc_grad,
@@ -350,9 +386,7 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
     int nx = NX_FINISH;
 
     // 1. the event
-    if (c.ev == EV_START) {
-        nx = NX_MAIN;
-    } else if (c.ev == EV_BOUNDARY) {
+    if (c.ev == EV_BOUNDARY) {
         if (c.kind == 1) {
             // exit: the sun disk is seen only by the camera ray (wgsl:745-749)
             if (c.depth == 0 && in_sun_cone(s, c.d)) {
@@ -368,31 +402,19 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
             c.nee_i = 0;
             c.nee_sum = 0.0f;
             nx = NX_NEE;
+        } else if (c.kind == 3) {
+            wd = true;
         }
-        // kind 2: tracking watchdog, the path ends
+        // kind 2: tracking watchdog or a main ray without boundary, the path ends
     } else if (c.ev == EV_HIT) {
-        const float hv = lane_altitude(s, c, c.t);
-        Coeffs co = lane_medium(s, c, hv);
+        // tp and rw were updated when the collision was accepted
         advance_vertex(c, c.t);
-        float scat = co.sr + co.sas;
-        float ext = scat + co.so + co.saa;
-        if constexpr (ADJ) {
-            // accepted collision times the albedo weight: d log(sigma_s / mu)
-            if (scat > 0.0f) add_dsigma_s(a.S, dsig(s, A, c.band, hv, c.bo, co), 1.0f / scat);
-            a.vsr = co.sr;
-            a.vsas = co.sas;
-            a.vh = hv;
-        }
-        if (ext > 0.0f && scat > 0.0f) {
-            c.tp = c.tp * fminf(fmaxf(f_div(scat, ext), 0.0f), 1.0f);
-            if (c.tp > 0.0f) {
-                c.vtype = 0;
-                c.vin = c.d;
-                c.rw = f_div(co.sr, scat);
-                c.nee_i = 0;
-                c.nee_sum = 0.0f;
-                nx = NX_NEE;
-            }
+        if (c.tp > 0.0f) {
+            c.vtype = 0;
+            c.vin = c.d;
+            c.nee_i = 0;
+            c.nee_sum = 0.0f;
+            nx = NX_NEE;
         }
     } else {  // EV_SHADOW
         c.nee_sum += c.nee_factor * c.aux;
@@ -455,7 +477,6 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
     // 4. weights
     bool need_ray = false;
     int mode = kTrackCollision;
-    double ox = c.px, oy = c.py, oz = c.pz;
     float factor = 0.0f;
     if (nx == NX_NEE) {
         if (c.vtype == 1) {
@@ -477,9 +498,6 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
             }
         }
         mode = (s.flags & kFlagRatio) ? kTrackRatio : kTrackDelta;
-        ox += (double)dir.x * kRayEpsilonKm;
-        oy += (double)dir.y * kRayEpsilonKm;
-        oz += (double)dir.z * kRayEpsilonKm;
         if (!need_ray) {
             // ground facing away from the sun: a zero sample
             c.st = ST_EVENT;
@@ -528,9 +546,6 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
                     c.px += (double)dir.x * kRayEpsilonKm;
                     c.py += (double)dir.y * kRayEpsilonKm;
                     c.pz += (double)dir.z * kRayEpsilonKm;
-                    ox = c.px;
-                    oy = c.py;
-                    oz = c.pz;
                     nx = NX_MAIN;
                 }
             }
@@ -550,31 +565,37 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
         }
     }
 
-    // 5. one ray set-up
     if (need_ray) {
-        bool hit = ray_begin(s, m, c, ox, oy, oz, dir, mode);
-        if (nx == NX_NEE) {
-            c.nee_factor = factor;
-            if (!(hit && c.kind == 1)) {
-                c.st = ST_EVENT;
-                c.ev = EV_SHADOW;
-                c.aux = 0.0f;
-            }
-        } else if (!hit) {
-            nx = NX_FINISH;
-        }
+        c.st = ST_RAY;
+        c.d = dir;
+        c.mode = mode;
+        if (nx == NX_NEE) c.nee_factor = factor;
     }
+    if (nx == NX_FINISH) finish_path<ADJ>(c, a, acc_sum, acc_sq, acc_grad);
+}
 
-    // 6. path end
-    if (nx == NX_FINISH) {
-        float Lf = fmaxf(c.L, 0.0f);
-        atomicAdd(&acc_sum[c.slot], Lf);
-        atomicAdd(&acc_sq[c.slot], Lf * Lf);
-        if constexpr (ADJ) {
-            for (int k = 0; k < kNP; ++k)
-                if (a.G[k] != 0.0f && a.G[k] == a.G[k]) atomicAdd(&acc_grad[k], a.G[k]);
+// Second half of a handler pass: one ray set-up for every lane in RAY -- a
+// shadow ray from the vertex moved by the ray epsilon (wgsl:744, 766) or a
+// main ray from the current origin (vertex already moved, or the camera).
+// A shadow ray that does not leave the atmosphere is a zero sun sample; a
+// main ray with no boundary ends the path at the next pass.
+__device__ __forceinline__ void ray_pass(const DevScene &s, const MajSrc &m, LaneCtx &c)
+{
+    const bool shadow = c.mode != kTrackCollision;
+    const f3 dir = c.d;
+    const double e = shadow ? (double)kRayEpsilonKm : 0.0;
+    bool hit = ray_begin(s, m, c, c.px + (double)dir.x * e, c.py + (double)dir.y * e, c.pz + (double)dir.z * e, dir,
+                         c.mode);
+    if (shadow) {
+        if (!(hit && c.kind == 1)) {
+            c.st = ST_EVENT;
+            c.ev = EV_SHADOW;
+            c.aux = 0.0f;
         }
-        c.st = ST_NEED;
+    } else if (!hit) {
+        c.st = ST_EVENT;
+        c.ev = EV_BOUNDARY;
+        c.kind = 2;
     }
 }
 
@@ -626,8 +647,18 @@ __device__ __forceinline__ void start_path(const DevScene &s, LaneCtx &c, AdjLan
     c.L = 0.0f;
     c.depth = 0;
     c.vtype = 0;
-    c.st = ST_EVENT;
-    c.ev = EV_START;
+    // main (wgsl:799-807): the camera ray, set up by the handler pass's ray
+    // stage.  At the loop top of trace_path only the depth watchdog can end
+    // a path at depth 0 (a max_depth of 0 means no cap).
+    c.d = c.vin;
+    c.mode = kTrackCollision;
+    if (s.watchdog_limit == 0) {
+        c.st = ST_EVENT;
+        c.ev = EV_BOUNDARY;
+        c.kind = 3;  // depth watchdog before the first ray
+    } else {
+        c.st = ST_RAY;
+    }
     if constexpr (ADJ) {
         const float *g = A.dl_dxyz + 3 * (size_t)pixel;
         a.w = (g[0] * l.wx + g[1] * l.wy + g[2] * l.wz) * A.inv_spp;
@@ -717,22 +748,25 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
         uint32_t next = 0;
         c.st = ST_NEED;
         for (;;) {
-            uint64_t need = ballot(c.st == ST_NEED);
-            if (need) {
-                if (c.st == ST_NEED) {
-                    uint32_t j = next + mbcnt(need);
-                    if (j < total)
-                        start_path<ADJ>(s, c, al, A, j, it);
-                    else
-                        c.st = ST_IDLE;
-                }
-                next += (uint32_t)__popcll(need);
-            }
-            uint64_t evm = ballot(c.st == ST_EVENT);
-            uint64_t trm = ballot(c.st == ST_TRACK);
-            if (!evm && !trm) break;
-            if (evm && ((uint32_t)__popcll(evm) >= a.event_threshold || !trm)) {
+            const uint64_t evm = ballot(c.st == ST_EVENT);
+            const uint64_t trm = ballot(c.st == ST_TRACK);
+            const uint64_t ndm = next < total ? ballot(c.st == ST_NEED) : 0ull;
+            if (!evm && !trm && !ndm) break;
+            if ((evm || ndm) && ((uint32_t)__popcll(evm | ndm) >= a.event_threshold || !trm)) {
+                // handler pass: events, refill of finished lanes, ray set-ups
                 if (c.st == ST_EVENT) handle_event<ADJ>(s, msrc, c, al, A, acc_sum, acc_sq, acc_grad, wd);
+                const uint64_t need = ballot(c.st == ST_NEED);
+                if (need && next < total) {
+                    if (c.st == ST_NEED) {
+                        uint32_t j = next + mbcnt(need);
+                        if (j < total)
+                            start_path<ADJ>(s, c, al, A, j, it);
+                        else
+                            c.st = ST_IDLE;
+                    }
+                    next += (uint32_t)__popcll(need);
+                }
+                if (c.st == ST_RAY) ray_pass(s, msrc, c);
             }
             for (uint32_t i = 0; i < a.track_steps; ++i) {
                 if (!ballot(c.st == ST_TRACK)) break;

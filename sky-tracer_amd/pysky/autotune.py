@@ -24,7 +24,7 @@ CACHE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_ca
 DEFAULT = dict(majorant_layers=16, event_threshold=32, track_steps=8, occupancy=0, waves_per_cu=16,
                pixels_per_item=0)
 SPACE = {
-    "majorant_layers": [8, 12, 16, 24, 32, 64],
+    "majorant_layers": [4, 5, 6, 7, 8, 12, 16, 24, 32, 64],
     "event_threshold": [8, 16, 24, 32, 40, 48, 56],
     "track_steps": [1, 2, 4, 8, 16, 32],
     "occupancy": [0, 4, 5, 6, 8],
