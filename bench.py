@@ -26,13 +26,17 @@ WORKLOADS = {
     "c2": dict(width=1024, height=512, n_lambda=64, spp=1024, max_depth=32, sun=5.0, albedo=0.3, aerosol=1.0),
     "c3": dict(width=512, height=256, n_lambda=64, spp=2048, max_depth=64, sun=-4.0, albedo=0.8, aerosol=4.0),
 }
-GRID_STEP = {"c1": 1, "c2": 16, "c3": 8}
 MODES = {
     "reference": dict(),
     "carried": dict(carried_tau=True),
     "carried_is": dict(carried_tau=True, phase_importance_sampling=True),
 }
-PEAK_FP32_TFLOPS = 157.3  # MI355X vector FP32 (packed); non-packed is half
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 (packed vector FMA); unpacked vector issue is half
+# HBM bytes per integrator launch from rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (scales from
+# a 256 MiB copy): profiles/k_wavepool_c2_1024x512x64_counters.md.  The launch writes one
+# [pixel][band] float2 partial per pixel group and reads only the scene tables, so the figure
+# does not depend on spp at the tuned work split.
+TRAFFIC_BYTES = {"c2": (2.0 * 105.9 + 270528.0) * 1024}
 
 
 def make_scene(w):
@@ -55,7 +59,8 @@ def make_config(w, mode, layers, seed=1):
 
 def cpu_baseline(wname, w, mode, layers, target_s=15.0):
     """The oracle (C restatement of the same estimator, FP32) on the host
-    cores, on the workload's subsampled pixel grid, scaled to ~target_s."""
+    cores: every sample of the workload for an evenly spread subset of its
+    pixels, the subset sized from a short probe so the run takes ~target_s."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
 
@@ -64,12 +69,9 @@ def cpu_baseline(wname, w, mode, layers, target_s=15.0):
     flags = {"reference": 0, "carried": O.FLAG_CARRIED_TAU,
              "carried_is": O.FLAG_CARRIED_TAU | O.FLAG_PHASE_IS}[mode]
     threads = max(1, min(os.cpu_count() or 1, 16))
-    step = GRID_STEP[wname]
-    xs = np.arange(step // 2, w["width"], step)
-    ys = np.arange(step // 2, w["height"], step)
-    pixels = (ys[:, None] * w["width"] + xs[None, :]).ravel()
+    npx = w["width"] * w["height"]
 
-    def run(spp):
+    def run(pixels, spp):
         p = O.Params(width=w["width"], height=w["height"], spp=spp, seed=1, sun_elevation_deg=w["sun"],
                      max_depth=w["max_depth"], ground_albedo=w["albedo"],
                      aerosol_sca_550_m_inv=3.996e-6 * w["aerosol"], aerosol_abs_550_m_inv=4.4e-6 * w["aerosol"],
@@ -78,17 +80,36 @@ def cpu_baseline(wname, w, mode, layers, target_s=15.0):
         O.render_pixels(p, pixels, "f32", threads=threads)
         return time.perf_counter() - t0, len(pixels) * w["n_lambda"] * spp
 
-    dt, n = run(2)  # warm-up and rate probe
-    spp = max(2, min(w["spp"], int(2 * target_s / max(dt, 1e-3))))
-    dt, n = run(spp)
+    probe = np.linspace(0, npx - 1, min(npx, 1024)).astype(np.int64)
+    dt, n = run(probe, min(w["spp"], 16))
+    rate = n / max(dt, 1e-6)
+    want = int(target_s * rate / (w["n_lambda"] * w["spp"]))
+    pixels = np.linspace(0, npx - 1, max(64, min(npx, want))).astype(np.int64)
+    dt, n = run(pixels, w["spp"])
     return {
         "value": n / dt,
         "unit": "path x lambda samples/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"oracle FP32, {len(pixels)} px (every {step}th of {w['width']}x{w['height']}) x "
-                  f"{w['n_lambda']} lambda x {spp} spp, {dt:.1f} s",
+        "sample": f"oracle FP32, {len(pixels)} of {w['width']}x{w['height']} px (evenly spread) x "
+                  f"{w['n_lambda']} lambda x {w['spp']} spp, {dt:.1f} s on {threads} threads",
     }
+
+
+def copy_calibration(mib):
+    """Three plain device-to-device copies of `mib` MiB: under rocprofv3 --pmc
+    their FETCH_SIZE / WRITE_SIZE per dispatch, against the known mib x 1024
+    KiB, give the scale factors applied to the integrator's counters."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_kib_per_dispatch": mib * 1024, "dispatches": 3}))
 
 
 def main():
@@ -102,7 +123,12 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="re-tune launch parameters first (untimed)")
     ap.add_argument("--spp", type=int, default=0, help="override samples per pixel")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 3 device copies of MIB MiB (scale for the FETCH/WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the integrator shards by sample range with no "
               "exchange but the final XYZ sum, and only the 1-GPU figure is reported.", file=sys.stderr)
@@ -157,8 +183,9 @@ def main():
         if key in fm["configs"]:
             f_unit = fm["configs"][key]["flops_per_sample"]
             achieved = f_unit * samples / avg_kernel_s / 1e12
-            roofline = {"bound": "valu-fp32", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-                        "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 5), "traffic": None,
+            roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                        "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 5),
+                        "traffic": TRAFFIC_BYTES.get(args.workload),
                         "flops_per_sample": round(f_unit, 1),
                         "frac_nonpacked": round(achieved / (PEAK_FP32_TFLOPS / 2), 5),
                         "rng_int_tops": round(fm["configs"][key]["rng_int_ops_per_sample"] * samples

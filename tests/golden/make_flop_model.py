@@ -74,7 +74,7 @@ def main():
     out = {"costs": COSTS, "int_ops_per_rng_draw": INT_OPS_PER_DRAW, "configs": {}}
     for wl in ("c1", "c2", "c3"):
         for mode in ("reference", "carried", "carried_is"):
-            layer_set = (8, 12, 16, 24, 32, 64) if mode == "carried_is" else (16, 64)
+            layer_set = (4, 5, 6, 7, 8, 12, 16, 24, 32, 64) if mode == "carried_is" else (16, 64)
             for layers in layer_set:
                 key = f"{wl}:{mode}:L{layers}"
                 out["configs"][key] = measure(wl, mode, layers)
