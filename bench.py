@@ -12,6 +12,7 @@ and the CPU oracle's throughput on the host cores for the same estimator.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -30,6 +31,7 @@ MODES = {
     "reference": dict(),
     "carried": dict(carried_tau=True),
     "carried_is": dict(carried_tau=True, phase_importance_sampling=True),
+    "carried_is_ratio": dict(carried_tau=True, phase_importance_sampling=True, ratio_tracking=True),
 }
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 (packed vector FMA); unpacked vector issue is half
 # HBM bytes per integrator launch from rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (scales from
@@ -67,7 +69,8 @@ def cpu_baseline(wname, w, mode, layers, target_s=15.0):
     import oracle as O
 
     flags = {"reference": 0, "carried": O.FLAG_CARRIED_TAU,
-             "carried_is": O.FLAG_CARRIED_TAU | O.FLAG_PHASE_IS}[mode]
+             "carried_is": O.FLAG_CARRIED_TAU | O.FLAG_PHASE_IS,
+             "carried_is_ratio": O.FLAG_CARRIED_TAU | O.FLAG_PHASE_IS | O.FLAG_RATIO_TRACKING}[mode]
     threads = max(1, min(os.cpu_count() or 1, 16))
     npx = w["width"] * w["height"]
 
@@ -112,12 +115,158 @@ def copy_calibration(mib):
     print(json.dumps({"copy_kib_per_dispatch": mib * 1024, "dispatches": 3}))
 
 
+def bench_batch(args, ctx):
+    """Ragged batch (256 renders: sun U[-6, 90] deg, 64x32..512x256, 3..64
+    wavelengths in 400-700 nm, 256 spp) in one persistent launch replayed
+    from a HIP graph, against the same renders issued one by one."""
+    import numpy as np
+    import torch
+
+    from pysky.batch import ragged_batch, render_batch
+    from pysky.torch_ops import render_device
+
+    batch = ragged_batch(n=256, seed=1, spp=args.spp or 256, max_depth=32)
+    layers = args.layers
+    batch = [(sc, dataclasses.replace(cfg, majorant_layers=layers)) for sc, cfg in batch]
+    samples = sum(cfg.width * cfg.height * cfg.spp * sc.band_count for sc, cfg in batch)
+    print(f"[bench] ragged batch: 256 renders, {samples:.3e} samples, layers={layers}", file=sys.stderr, flush=True)
+    for i in range(args.warmup):
+        r = render_batch(batch, context=ctx)
+        print(f"[bench] warmup {i}: kernel {r.stats.kernel_ms:.1f} ms", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    kms, p50, p99 = [], [], []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        r = render_batch(batch, context=ctx)
+        kms.append(r.stats.kernel_ms)
+        p50.append(r.stats.latency_p50_ms)
+        p99.append(r.stats.latency_p99_ms)
+        print(f"[bench] step {i}: kernel {r.stats.kernel_ms:.1f} ms graph_reused={r.stats.graph_reused} "
+              f"p50 {r.stats.latency_p50_ms:.1f} p99 {r.stats.latency_p99_ms:.1f} ms", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    # the same renders one at a time (device outputs, no host copies)
+    dev = torch.device("cuda:0")
+    outs = [torch.zeros(cfg.height * cfg.width * 3, dtype=torch.float32, device=dev) for _, cfg in batch]
+    for (sc, cfg), o in zip(batch[:8], outs[:8]):
+        render_device(sc, cfg, xyz=o, context=ctx)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    single_kernel_ms = 0.0
+    for (sc, cfg), o in zip(batch, outs):
+        single_kernel_ms += render_device(sc, cfg, xyz=o, context=ctx).kernel_ms
+    torch.cuda.synchronize()
+    single_wall = time.perf_counter() - t1
+    value = samples / wall
+    single_rate = samples / single_wall
+    return {
+        "metric": "ragged-batch path x lambda samples/s on 1 MI355X",
+        "value": value,
+        "unit": "path x lambda samples/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 ray set-up)",
+        "data": "synthetic (seeded ragged batch, analytic atmosphere)",
+        "config": {"workload": "c4: 256 ragged renders (sun U[-6,90] deg, 64x32..512x256, 3..64 lambda in "
+                               "400-700 nm, 256 spp, depth<=32), one launch, HIP graph",
+                   "estimator": "carried_is", "majorant_layers": layers},
+        "kernel_ms": float(np.mean(kms)),
+        "latency_p50_ms": float(np.mean(p50)), "latency_p99_ms": float(np.mean(p99)),
+        "single_render_rate": single_rate, "single_render_kernel_ms_sum": single_kernel_ms,
+        "batch_over_single": value / single_rate,
+    }
+
+
+def bench_adjoint(args, ctx):
+    """Adjoint at 256x128, 32 lambda, 256 spp (sun 10 deg, depth <= 8): one
+    gradient evaluation = forward render + path replay.  The gradient's
+    relative RMSE at 256 spp comes from 8 independent seeds; the spp that
+    reaches 1% follows from RMSE ~ spp^-1/2."""
+    import numpy as np
+    import torch
+
+    import pysky
+    from pysky.adjoint import render_adjoint_device
+    from pysky.torch_ops import render_device
+
+    spp = args.spp or 256
+    scene = pysky.SceneData(sun_elevation_deg=10.0, wavelengths_nm=list(pysky.midpoint_grid(32)))
+    base = pysky.RenderConfig(width=256, height=128, spp=spp, seed=7, max_depth=8, rr_start_depth=4,
+                              majorant_layers=args.layers or 8, **MODES[args.mode])
+    dev = torch.device("cuda:0")
+    xyz = torch.zeros(128 * 256 * 3, dtype=torch.float32, device=dev)
+    render_device(scene, base, xyz=xyz, context=ctx)
+    cot = (2.0 * xyz / xyz.numel()).contiguous()  # d/dXYZ of sum(XYZ^2) / n
+    grads, f_ms, a_ms = [], [], []
+    for i in range(max(args.warmup, 1) + 8):
+        cfg = dataclasses.replace(base, seed=100 + i)
+        st = render_device(scene, cfg, xyz=xyz, context=ctx)
+        g, ast = render_adjoint_device(scene, cfg, cot, context=ctx)
+        if i >= max(args.warmup, 1):
+            grads.append(g)
+            f_ms.append(st.kernel_ms)
+            a_ms.append(ast.kernel_ms)
+    torch.cuda.synchronize()
+    G = np.array(grads)
+    mean = G.mean(axis=0)
+    rel_rmse = float(np.sqrt(G.var(axis=0, ddof=1).sum()) / np.linalg.norm(mean))
+    spp_1pct = int(np.ceil(spp * (rel_rmse / 0.01) ** 2))
+    t_eval = (np.mean(f_ms) + np.mean(a_ms)) / 1e3
+    evals_1pct = 1.0 / (t_eval * spp_1pct / spp)
+    return {
+        "metric": "adjoint grad evals/s at 1% RMSE on 1 MI355X",
+        "value": evals_1pct,
+        "unit": "gradient evaluations/s",
+        "n_gpus": 1, "steps": 8, "warmup": max(args.warmup, 1), "ms_per_step": t_eval * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 ray set-up)",
+        "data": "synthetic (analytic atmosphere)",
+        "config": {"workload": f"c5: 256x128, 32 lambda, {spp} spp, depth<=8, sun 10 deg, 10-parameter adjoint",
+                   "estimator": args.mode, "majorant_layers": base.majorant_layers},
+        "forward_ms": float(np.mean(f_ms)), "adjoint_ms": float(np.mean(a_ms)),
+        "adjoint_over_forward": float(np.mean(a_ms) / np.mean(f_ms)),
+        "grad_rel_rmse_at_spp": rel_rmse, "spp_for_1pct": spp_1pct,
+        "grad_evals_per_s_at_spp": 1.0 / t_eval,
+    }
+
+
+def run_fit(args, ctx):
+    """Recover +/-20%-perturbed parameters from a seed-7 target render by Adam
+    (200 steps) on the C5 configuration."""
+    import numpy as np
+    import torch
+
+    import pysky
+    from pysky import fit
+    from pysky.adjoint import params_vector
+    from pysky.torch_ops import render_device
+
+    scene = pysky.SceneData(sun_elevation_deg=10.0, wavelengths_nm=list(pysky.midpoint_grid(32)))
+    cfg = pysky.RenderConfig(width=256, height=128, spp=args.spp or 256, seed=7, max_depth=8, rr_start_depth=4,
+                             carried_tau=True, phase_importance_sampling=True, majorant_layers=args.layers or 8)
+    target = torch.zeros((128, 256, 3), dtype=torch.float32, device="cuda:0")
+    render_device(scene, cfg, xyz=target, context=ctx)
+    truth = params_vector(scene.atmosphere)
+    theta0 = fit.perturbed(scene)
+    res = fit.fit(scene, cfg, target, theta0, steps=200, context=ctx,
+                  log=lambda m: print(m, file=sys.stderr, flush=True))
+    err0 = np.abs(theta0 / truth - 1.0)
+    err = np.abs(res["theta"] / truth - 1.0)
+    return {
+        "metric": "inverse fit (Adam, 200 steps, adjoint gradients) on 1 MI355X",
+        "value": res["seconds"], "unit": "s", "higher_is_better": False, "n_gpus": 1,
+        "config": {"workload": "c5 fit: 256x128, 32 lambda, 256 spp, target seed 7, start +/-20%"},
+        "loss_first": res["history"][0], "loss_last": float(np.mean(res["history"][-10:])),
+        "param_names": list(pysky.PARAM_NAMES),
+        "rel_error_start": [round(float(v), 4) for v in err0],
+        "rel_error_end": [round(float(v), 4) for v in err],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + ["c4", "c5", "c5fit"])
     ap.add_argument("--mode", default="carried_is", choices=sorted(MODES))
     ap.add_argument("--layers", type=int, default=0, help="majorant shells (0: tuned)")
     ap.add_argument("--autotune", action="store_true", help="re-tune launch parameters first (untimed)")
@@ -140,6 +289,22 @@ def main():
     from pysky.torch_ops import render_device
 
     from pysky import autotune
+
+    if args.workload in ("c4", "c5", "c5fit"):
+        ctx = pysky.Context(0)
+        if args.workload == "c4":
+            p = autotune.tuned(pysky.SceneData(wavelengths_nm=list(pysky.midpoint_grid(64))),
+                               pysky.RenderConfig(width=1024, height=512, carried_tau=True,
+                                                  phase_importance_sampling=True))
+            autotune.apply(ctx, pysky.RenderConfig(), p)
+            args.layers = args.layers or p["majorant_layers"]
+            out = bench_batch(args, ctx)
+        elif args.workload == "c5":
+            out = bench_adjoint(args, ctx)
+        else:
+            out = run_fit(args, ctx)
+        print(json.dumps(out))
+        return
 
     w = dict(WORKLOADS[args.workload])
     if args.spp:
