@@ -178,9 +178,9 @@ def bench_batch(args, ctx):
 
 def bench_adjoint(args, ctx):
     """Adjoint at 256x128, 32 lambda, 256 spp (sun 10 deg, depth <= 8): one
-    gradient evaluation = forward render + path replay.  The gradient's
-    relative RMSE at 256 spp comes from 8 independent seeds; the spp that
-    reaches 1% follows from RMSE ~ spp^-1/2."""
+    gradient evaluation = forward render + path replay of d mean(log XYZ) /
+    d theta.  The gradient's relative RMSE at 256 spp comes from 8
+    independent seeds; the spp that reaches 1% follows from RMSE ~ spp^-1/2."""
     import numpy as np
     import torch
 
@@ -194,8 +194,11 @@ def bench_adjoint(args, ctx):
                               majorant_layers=args.layers or 8, **MODES[args.mode])
     dev = torch.device("cuda:0")
     xyz = torch.zeros(128 * 256 * 3, dtype=torch.float32, device=dev)
-    render_device(scene, base, xyz=xyz, context=ctx)
-    cot = (2.0 * xyz / xyz.numel()).contiguous()  # d/dXYZ of sum(XYZ^2) / n
+    # loss = mean(log XYZ) over the image, so every pixel weighs by its relative
+    # change (an XYZ^2 loss would be the sun disk's transmittance alone);
+    # its cotangent 1 / (n XYZ) comes from an independent 4x-spp render
+    render_device(scene, dataclasses.replace(base, spp=4 * spp, seed=99), xyz=xyz, context=ctx)
+    cot = (1.0 / (xyz.numel() * torch.clamp(xyz, min=1e-3 * float(xyz.mean())))).contiguous()
     grads, f_ms, a_ms = [], [], []
     for i in range(max(args.warmup, 1) + 8):
         cfg = dataclasses.replace(base, seed=100 + i)
@@ -209,7 +212,9 @@ def bench_adjoint(args, ctx):
     G = np.array(grads)
     mean = G.mean(axis=0)
     rel_rmse = float(np.sqrt(G.var(axis=0, ddof=1).sum()) / np.linalg.norm(mean))
-    spp_1pct = int(np.ceil(spp * (rel_rmse / 0.01) ** 2))
+    per_param = np.sqrt(G.var(axis=0, ddof=1)) / np.maximum(np.abs(mean), 1e-300)
+    # spp that reaches 1% (RMSE ~ spp^-1/2), never below the configuration's own
+    spp_1pct = max(spp, int(np.ceil(spp * (rel_rmse / 0.01) ** 2)))
     t_eval = (np.mean(f_ms) + np.mean(a_ms)) / 1e3
     evals_1pct = 1.0 / (t_eval * spp_1pct / spp)
     return {
@@ -224,6 +229,7 @@ def bench_adjoint(args, ctx):
         "forward_ms": float(np.mean(f_ms)), "adjoint_ms": float(np.mean(a_ms)),
         "adjoint_over_forward": float(np.mean(a_ms) / np.mean(f_ms)),
         "grad_rel_rmse_at_spp": rel_rmse, "spp_for_1pct": spp_1pct,
+        "grad_rel_std_per_param": [round(float(v), 5) for v in per_param],
         "grad_evals_per_s_at_spp": 1.0 / t_eval,
     }
 
