@@ -35,13 +35,19 @@ def fit(scene: SceneData, config: RenderConfig, target_xyz, theta0, steps: int =
         log: Optional[Callable[[str], None]] = None) -> dict:
     """Returns dict(theta, history, seconds).  target_xyz: float32 CUDA
     tensor [H, W, 3]; theta0: starting parameters (sky_hip.h order).  The
-    loss is the mean squared relative error of block means, so the sun disk
-    (10^4 times brighter than the sky) does not outweigh the sky."""
+    loss is the mean squared relative error of block means outside the sun
+    disk, so the direct sun (10^4 times brighter than the sky) does not
+    outweigh the sky."""
     import torch
 
     dev = target_xyz.device
     tb = _blocks(target_xyz.double(), block)
     scale = tb + 1e-3 * float(tb.mean())
+    # blocks that contain the sun disk: a rare, huge direct term whose Monte
+    # Carlo noise would drown the sky's signal; they are left out of the loss
+    lum = tb[..., 1]
+    keep = (lum < 20.0 * lum.median()).to(torch.float64)[..., None]
+    n_keep = float(keep.sum()) * tb.shape[-1]
     u = torch.tensor(np.log(np.abs(np.asarray(theta0, dtype=np.float64))), dtype=torch.float64, device=dev,
                      requires_grad=True)
     sign = torch.tensor(np.sign(np.asarray(theta0, dtype=np.float64)), dtype=torch.float64, device=dev)
@@ -57,7 +63,7 @@ def fit(scene: SceneData, config: RenderConfig, target_xyz, theta0, steps: int =
         opt.zero_grad()
         theta = sign * torch.exp(u)
         xyz = sky_render(theta, scene, cfg, context)
-        loss = (((_blocks(xyz.double(), block) - tb) / scale) ** 2).mean()
+        loss = ((((_blocks(xyz.double(), block) - tb) / scale) ** 2) * keep).sum() / n_keep
         loss.backward()
         opt.step()
         sched.step()
