@@ -349,6 +349,8 @@ def main():
     roofline = None
     fm_path = os.path.join(ROOT, "sky-tracer_amd", "pysky", "flop_model.json")
     key = f"{args.workload}:{args.mode}:L{args.layers}"
+    if cfg.shell_growth != 1.0:
+        key += f":g{cfg.shell_growth:g}"
     if os.path.exists(fm_path):
         fm = json.load(open(fm_path))
         if key in fm["configs"]:

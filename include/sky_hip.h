@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define SKY_HIP_ABI_VERSION 1
+#define SKY_HIP_ABI_VERSION 2
 
 /* RenderError (integrator.rs:19-26) as status codes. */
 typedef enum sky_status {
@@ -104,6 +104,12 @@ typedef struct sky_render_config {
      * computed majorant grid, e.g. to hold it fixed while parameters move
      * (finite differences with common random numbers, the adjoint). */
     const float *majorant_table;
+    /* Majorant shell thickness growth from the ground up: 1 (or 0) = the
+     * reference's uniform shells; g > 1 gives h_k = top (g^k-1)/(g^L-1),
+     * thin shells where the density changes fastest.  Any shells sample the
+     * same free-flight law; this only moves cost between shell crossings and
+     * null collisions. */
+    double shell_growth;
 } sky_render_config;
 
 typedef struct sky_render_stats {
@@ -142,10 +148,11 @@ void sky_default_config(sky_render_config *cfg);
  * sun_radiance (W m^-2 sr^-1 nm^-1), wx, wy, wz]; n_out >= 8 * count. */
 sky_status sky_wavelength_table(const sky_atmosphere *atm, const sky_wavelengths *wl, float *out,
                                 uint32_t n_out);
-/* Majorant grid (medium.rs:63-83): [lambda * layers + layer], km^-1. */
+/* Majorant grid (medium.rs:63-83): [lambda * layers + layer], km^-1, over
+ * the shells of sky_render_config.shell_growth. */
 sky_status sky_majorant_grid(const sky_planet *planet, const sky_atmosphere *atm,
-                             const sky_wavelengths *wl, uint32_t layers, double margin, float *out,
-                             uint32_t n_out);
+                             const sky_wavelengths *wl, uint32_t layers, double margin,
+                             double shell_growth, float *out, uint32_t n_out);
 
 /* validate_config (integrator.rs:130-163) for the generalised inputs:
  * SKY_OK or SKY_ERR_INVALID_CONFIG with the reason in sky_last_error(). */

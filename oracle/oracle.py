@@ -63,6 +63,7 @@ class _Params(C.Structure):
         ("n_lambda", C.c_uint32),
         ("lambda_nm", C.POINTER(C.c_double)),
         ("weight_nm", C.POINTER(C.c_double)),
+        ("shell_growth", C.c_double),
     ]
 
 
@@ -198,6 +199,7 @@ class Params:
     majorant_margin: float = 1.05
     lambda_nm: Sequence[float] = field(default_factory=lambda: list(midpoint_grid(16)))
     weight_nm: Optional[Sequence[float]] = None
+    shell_growth: float = 1.0
 
     def to_c(self):
         p = _Params()
@@ -276,6 +278,15 @@ def coefficients(p: Params, band: int, h: float) -> np.ndarray:
     out = [C.c_double() for _ in range(4)]
     lib().orc_coefficients(C.byref(cp), C.byref(lam), h, *[C.byref(o) for o in out])
     return np.array([o.value for o in out])
+
+
+def shell_heights(p: Params) -> np.ndarray:
+    """Shell boundary heights h_0..h_L (km)."""
+    cp = p.to_c()
+    f = lib().orc_shell_height
+    f.restype = C.c_double
+    f.argtypes = [C.POINTER(_Params), C.c_int]
+    return np.array([f(C.byref(cp), k) for k in range(p.majorant_layers + 1)])
 
 
 def majorant_grid(p: Params) -> np.ndarray:

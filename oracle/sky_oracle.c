@@ -129,6 +129,7 @@ void orc_default_params(orc_params *p)
     p->max_segments = 100000;   /* sky_trace.wgsl:412 */
     p->majorant_layers = 64;    /* data.rs:48 */
     p->majorant_margin = 1.05;  /* medium.rs:77 */
+    p->shell_growth = 1.0;      /* uniform shells, as the reference */
 }
 
 /* Per-wavelength coefficients of the analytic medium (data.rs:26-33 and
@@ -187,6 +188,27 @@ void orc_coefficients(const orc_params *p, const orc_lambda *lam, double h, doub
     *saa = lam->beta_aa * ra;
 }
 
+/* Shell boundaries: h_k = top k / L (uniform), else
+ * h_k = top (g^k - 1) / (g^L - 1) for growth g > 1. */
+double orc_shell_height(const orc_params *p, int k)
+{
+    double top = p->atmosphere_radius_km - p->ground_radius_km;
+    int L = (int)p->majorant_layers;
+    double g = p->shell_growth;
+    if (k <= 0) return 0.0;
+    if (k >= L) return top;
+    if (!(g > 1.0)) return k * (top / L);
+    return top * (pow(g, k) - 1.0) / (pow(g, L) - 1.0);
+}
+
+int orc_shell_of(const orc_params *p, double h)
+{
+    int L = (int)p->majorant_layers;
+    int k = 0;
+    while (k < L - 1 && orc_shell_height(p, k + 1) <= h) ++k;
+    return k;
+}
+
 /* Majorant grid, medium.rs:63-83: margin * max extinction over each shell,
  * floored at 1e-8 (medium.rs:77, sky_trace.wgsl:260).  For these profiles the
  * maximum is exact: both exponentials peak at the shell's lower edge and the
@@ -197,11 +219,9 @@ int orc_majorant_grid(const orc_params *p, double *out)
     if (orc_lambda_table(p, lam)) return 1;
     uint32_t L = p->majorant_layers;
     if (L == 0 || L > ORC_MAX_LAYERS) return 1;
-    double top = p->atmosphere_radius_km - p->ground_radius_km;
-    double dz = top / L;
     for (uint32_t b = 0; b < p->n_lambda; ++b) {
         for (uint32_t k = 0; k < L; ++k) {
-            double lo = k * dz, hi = (k + 1) * dz;
+            double lo = orc_shell_height(p, (int)k), hi = orc_shell_height(p, (int)k + 1);
             double d = 0.0;
             if (p->ozone_center_km < lo) d = lo - p->ozone_center_km;
             if (p->ozone_center_km > hi) d = p->ozone_center_km - hi;
@@ -327,17 +347,12 @@ typedef struct walk_ray {
 static void walk_init(const orc_params *p, const double o[3], const double d[3], walk_ray *w)
 {
     double R = p->ground_radius_km;
-    double top = p->atmosphere_radius_km - R;
-    int L = (int)p->majorant_layers;
-    double dz = top / L;
     w->b = dot3(o, d);
     w->q = dot3(o, o) - w->b * w->b - R * R;
     double h0 = sqrt(dot3(o, o)) - R;
-    int k = (int)floor(h0 / dz);
+    int k = orc_shell_of(p, h0);
     w->outward = w->b >= 0.0;
-    if (!w->outward && k > 0 && h0 - k * dz <= 1e-9) k -= 1;
-    if (k < 0) k = 0;
-    if (k > L - 1) k = L - 1;
+    if (!w->outward && k > 0 && h0 - orc_shell_height(p, k) <= 1e-9) k -= 1;
     w->layer = k;
 }
 
@@ -345,9 +360,8 @@ static double walk_next(const orc_params *p, const walk_ray *w, int k, int out, 
 {
     double R = p->ground_radius_km;
     int L = (int)p->majorant_layers;
-    double dz = (p->atmosphere_radius_km - R) / L;
     if (!out) {
-        double hk = k * dz;
+        double hk = orc_shell_height(p, k);
         double disc = hk * (2.0 * R + hk) - w->q;
         if (disc > 0.0) {
             *on = 0;
@@ -365,7 +379,7 @@ static double walk_next(const orc_params *p, const walk_ray *w, int k, int out, 
         *kn = k;
         return INFINITY;
     }
-    double hk = (k + 1) * dz;
+    double hk = orc_shell_height(p, k + 1);
     double disc = hk * (2.0 * R + hk) - w->q;
     return sqrt(disc > 0.0 ? disc : 0.0) - w->b;
 }
@@ -560,6 +574,8 @@ static int validate(const orc_params *p)
     if ((uint64_t)p->width * p->height * (p->n_lambda ? p->n_lambda : 1) > 0xFFFFFFFFull) return 1;
     if (p->n_lambda == 0 || p->n_lambda > ORC_MAX_LAMBDA || !p->lambda_nm) return 1;
     if (p->majorant_layers == 0 || p->majorant_layers > ORC_MAX_LAYERS) return 1;
+    if (!(p->shell_growth >= 1.0 && p->shell_growth <= 16.0)) return 1;
+    if (p->majorant_layers * log2(p->shell_growth) > 1000.0) return 1;
     if (!(p->rayleigh_h_km > 0) || !(p->aerosol_h_km > 0) || !(p->ozone_half_width_km > 0)) return 1;
     if (!(p->aerosol_g > -1 && p->aerosol_g < 1)) return 1;
     if (!(p->atmosphere_radius_km > p->ground_radius_km) || !(p->ground_radius_km > 0)) return 1;

@@ -86,6 +86,11 @@ typedef struct orc_params {
     uint32_t n_lambda;
     const double *lambda_nm;
     const double *weight_nm;
+    /* Majorant shell thicknesses grow by this factor from the ground up
+     * (1: the reference's uniform shells, medium.rs:63-83).  Any shells give
+     * the same delta-tracking law; thin shells near the ground, where the
+     * density falls fastest, give tighter majorants for the same count. */
+    double shell_growth;
 } orc_params;
 
 /* Per-wavelength constants derived from the parameters. */
@@ -94,6 +99,11 @@ typedef struct orc_lambda {
     double sun_radiance;                     /* W m^-2 sr^-1 nm^-1            */
     double wx, wy, wz;                       /* cmf(lambda) * weight_nm       */
 } orc_lambda;
+
+/* Height of the lower boundary of shell k (k = 0..L; h_L = top). */
+double orc_shell_height(const orc_params *p, int k);
+/* Shell containing altitude h: the largest k < L with h_k <= h. */
+int orc_shell_of(const orc_params *p, double h);
 
 /* RNG: pcg4d counter hash (sky_trace.wgsl:156-189). */
 void orc_pcg4d(const uint32_t in[4], uint32_t out[4]);

@@ -66,6 +66,7 @@ Film render(const SceneData &scene, const RenderConfig &config)
     c.max_depth = config.max_depth;
     c.rr_start_depth = config.rr_start_depth;
     c.majorant_layers = config.majorant_layers;
+    c.shell_growth = config.shell_growth;
     c.flags = config.flags;
     sky_wavelengths wl;
     wl.count = (uint32_t)scene.wavelengths_nm.size();

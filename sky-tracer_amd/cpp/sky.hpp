@@ -35,6 +35,7 @@ struct RenderConfig {  // config.rs:2-12, defaults config.rs:38-52
     uint32_t sample_offset = 0, max_depth = 0, rr_start_depth = 4;
     uint32_t majorant_layers = 64;
     uint32_t flags = 0;
+    double shell_growth = 1.0;  // majorant shell thickness growth (1: uniform)
 };
 
 struct SceneData {  // analytic counterpart of sky_core::atmosphere::SceneData

@@ -57,10 +57,11 @@ struct sky_context {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     sky_tuning tuning{};
-    DevBuf lam, maj, film, film_m2, xyz, accum, meta, diag, rays, tau, lamd, jobs, adj_units, adj_dl, adj_grad;
+    DevBuf lam, maj, shells, film, film_m2, xyz, accum, meta, diag, rays, tau, lamd, jobs, adj_units, adj_dl,
+        adj_grad;
     std::vector<sky::DevJob> host_jobs;  // staging, alive until the stream syncs
     // ragged batches
-    DevBuf b_lam, b_maj, b_xyz, b_film, b_accum, b_meta, b_jobs;
+    DevBuf b_lam, b_maj, b_shells, b_xyz, b_film, b_accum, b_meta, b_jobs;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     uint64_t graph_key = 0;
@@ -87,6 +88,7 @@ sky_status run_render(sky_context *ctx, sky::HostScene &hs, hipStream_t stream,
     sky::DevScene &s = hs.dev;
     HIP_TRY(ctx->lam.ensure(hs.lam.size() * sizeof(sky::DevLambda)), "alloc lambda table", SKY_ERR_OOM);
     HIP_TRY(ctx->maj.ensure(hs.maj.size() * sizeof(float)), "alloc majorants", SKY_ERR_OOM);
+    HIP_TRY(ctx->shells.ensure(hs.shell_d.size() * sizeof(double)), "alloc shell table", SKY_ERR_OOM);
     HIP_TRY(ctx->diag.ensure(64), "alloc diagnostics", SKY_ERR_OOM);
     HIP_TRY(ctx->meta.ensure(64), "alloc work counters", SKY_ERR_OOM);
     HIP_TRY(ctx->jobs.ensure(sizeof(sky::DevJob)), "alloc job table", SKY_ERR_OOM);
@@ -96,9 +98,13 @@ sky_status run_render(sky_context *ctx, sky::HostScene &hs, hipStream_t stream,
     HIP_TRY(hipMemcpyAsync(ctx->maj.p, hs.maj.data(), hs.maj.size() * sizeof(float),
                            hipMemcpyHostToDevice, stream),
             "upload majorants", SKY_ERR_COPY);
+    HIP_TRY(hipMemcpyAsync(ctx->shells.p, hs.shell_d.data(), hs.shell_d.size() * sizeof(double),
+                           hipMemcpyHostToDevice, stream),
+            "upload shell table", SKY_ERR_COPY);
     HIP_TRY(sky::launch_zero((uint32_t *)ctx->diag.p, 16, stream), "clear diagnostics", SKY_ERR_SYNC);
     s.lam = (const sky::DevLambda *)ctx->lam.p;
     s.maj = (const float *)ctx->maj.p;
+    s.shell_d = (const double *)ctx->shells.p;
     s.diag = (uint32_t *)ctx->diag.p;
     const bool simple = (s.flags & SKY_FLAG_SIMPLE_KERNEL) != 0;
     sky::LaunchShape shape;
@@ -116,7 +122,7 @@ sky_status run_render(sky_context *ctx, sky::HostScene &hs, hipStream_t stream,
         j.t_done = (unsigned long long *)((uint32_t *)ctx->meta.p + 6);
         HIP_TRY(hipMemcpyAsync(ctx->jobs.p, &j, sizeof(j), hipMemcpyHostToDevice, stream), "upload job",
                 SKY_ERR_COPY);
-        plan = sky::plan_launch(ctx->tuning, ctx->num_cus, j.n_items, s.nlam * s.L);
+        plan = sky::plan_launch(ctx->tuning, ctx->num_cus, j.n_items, s.nlam * s.L, s.L);
     } else if (!s.film) {
         size_t bytes = (size_t)npx * s.nlam * sizeof(float);
         HIP_TRY(ctx->film.ensure(bytes), "alloc film", SKY_ERR_OOM);
@@ -205,6 +211,7 @@ void sky_default_config(sky_render_config *c)
     c->majorant_layers = 64;
     c->flags = 0;
     c->majorant_margin = 1.05;
+    c->shell_growth = 1.0;
 }
 
 sky_status sky_context_create(int device, sky_context **out)
@@ -240,7 +247,7 @@ void sky_context_destroy(sky_context *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
-    for (DevBuf *b : {&c->lam, &c->maj, &c->film, &c->film_m2, &c->xyz, &c->accum, &c->meta, &c->diag,
+    for (DevBuf *b : {&c->lam, &c->maj, &c->shells, &c->b_shells, &c->film, &c->film_m2, &c->xyz, &c->accum, &c->meta, &c->diag,
                       &c->rays, &c->tau, &c->lamd, &c->jobs, &c->b_lam, &c->b_maj, &c->b_xyz, &c->b_film,
                       &c->b_accum, &c->b_meta, &c->b_jobs, &c->adj_units, &c->adj_dl, &c->adj_grad})
         b->release();
@@ -280,15 +287,16 @@ sky_status sky_wavelength_table(const sky_atmosphere *atm, const sky_wavelengths
 }
 
 sky_status sky_majorant_grid(const sky_planet *planet, const sky_atmosphere *atm,
-                             const sky_wavelengths *wl, uint32_t layers, double margin, float *out,
-                             uint32_t n_out)
+                             const sky_wavelengths *wl, uint32_t layers, double margin,
+                             double shell_growth, float *out, uint32_t n_out)
 {
+    const double g = shell_growth == 0.0 ? 1.0 : shell_growth;
     if (!planet || !atm || !wl || !out || !wl->center_nm || wl->count == 0 || wl->count > 64 ||
-        layers == 0 || layers > 256)
+        layers == 0 || layers > 256 || !(g >= 1.0 && g <= 16.0) || layers * std::log2(g) > 1000.0)
         return fail(SKY_ERR_INVALID_CONFIG, "invalid majorant grid request");
     if (n_out < layers * wl->count) return fail(SKY_ERR_INVALID_CONFIG, "output too small");
     std::vector<float> maj;
-    sky::fill_majorants(planet, atm, wl, layers, margin, maj);
+    sky::fill_majorants(planet, atm, wl, layers, margin, g, maj);
     std::memcpy(out, maj.data(), maj.size() * sizeof(float));
     return SKY_OK;
 }
@@ -384,6 +392,7 @@ sky_status adjoint_impl(sky_context *ctx, const sky_planet *planet, const sky_at
     sky::DevScene &s = hs.dev;
     HIP_TRY(ctx->lam.ensure(hs.lam.size() * sizeof(sky::DevLambda)), "alloc lambda table", SKY_ERR_OOM);
     HIP_TRY(ctx->maj.ensure(hs.maj.size() * sizeof(float)), "alloc majorants", SKY_ERR_OOM);
+    HIP_TRY(ctx->shells.ensure(hs.shell_d.size() * sizeof(double)), "alloc shell table", SKY_ERR_OOM);
     HIP_TRY(ctx->diag.ensure(64), "alloc diagnostics", SKY_ERR_OOM);
     HIP_TRY(ctx->meta.ensure(64), "alloc work counters", SKY_ERR_OOM);
     HIP_TRY(ctx->jobs.ensure(sizeof(sky::DevJob)), "alloc job table", SKY_ERR_OOM);
@@ -399,6 +408,8 @@ sky_status adjoint_impl(sky_context *ctx, const sky_planet *planet, const sky_at
                            hipMemcpyHostToDevice, stream), "upload lambda table", SKY_ERR_COPY);
     HIP_TRY(hipMemcpyAsync(ctx->maj.p, hs.maj.data(), hs.maj.size() * sizeof(float), hipMemcpyHostToDevice,
                            stream), "upload majorants", SKY_ERR_COPY);
+    HIP_TRY(hipMemcpyAsync(ctx->shells.p, hs.shell_d.data(), hs.shell_d.size() * sizeof(double),
+                           hipMemcpyHostToDevice, stream), "upload shell table", SKY_ERR_COPY);
     HIP_TRY(hipMemcpyAsync(ctx->adj_units.p, units.data(), units.size() * sizeof(float4), hipMemcpyHostToDevice,
                            stream), "upload adjoint table", SKY_ERR_COPY);
     HIP_TRY(sky::launch_zero((uint32_t *)ctx->diag.p, 16, stream), "clear diagnostics", SKY_ERR_SYNC);
@@ -406,6 +417,7 @@ sky_status adjoint_impl(sky_context *ctx, const sky_planet *planet, const sky_at
             SKY_ERR_SYNC);
     s.lam = (const sky::DevLambda *)ctx->lam.p;
     s.maj = (const float *)ctx->maj.p;
+    s.shell_d = (const double *)ctx->shells.p;
     s.diag = (uint32_t *)ctx->diag.p;
     s.film = s.film_m2 = s.xyz = nullptr;
     ctx->host_jobs.assign(1, sky::DevJob{});
@@ -418,7 +430,7 @@ sky_status adjoint_impl(sky_context *ctx, const sky_planet *planet, const sky_at
     j.done = (uint32_t *)ctx->meta.p + 4;
     j.t_done = (unsigned long long *)((uint32_t *)ctx->meta.p + 6);
     HIP_TRY(hipMemcpyAsync(ctx->jobs.p, &j, sizeof(j), hipMemcpyHostToDevice, stream), "upload job", SKY_ERR_COPY);
-    sky::WavePlan plan = sky::plan_launch(ctx->tuning, ctx->num_cus, j.n_items, s.nlam * s.L);
+    sky::WavePlan plan = sky::plan_launch(ctx->tuning, ctx->num_cus, j.n_items, s.nlam * s.L, s.L);
     sky::AdjArgs A;
     A.dl_dxyz = d_dl;
     A.lam_units = (const float4 *)ctx->adj_units.p;
@@ -519,9 +531,9 @@ sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint3
     std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost(a) > cost(b); });
 
     // Table offsets.
-    size_t n_lam = 0, n_maj = 0, n_xyz = 0, n_film = 0, n_acc = 0;
-    uint32_t max_npx = 0, total_items = 0, max_maj = 0;
-    std::vector<size_t> o_lam(n), o_maj(n), o_xyz(n), o_film(n), o_acc(n);
+    size_t n_lam = 0, n_maj = 0, n_shl = 0, n_xyz = 0, n_film = 0, n_acc = 0;
+    uint32_t max_npx = 0, total_items = 0, max_maj = 0, max_layers = 0;
+    std::vector<size_t> o_lam(n), o_maj(n), o_shl(n), o_xyz(n), o_film(n), o_acc(n);
     ctx->host_jobs.assign(n, sky::DevJob{});
     const uint32_t min_items = 0;  // the batch balances across renders; no sample split needed
     uint64_t samples = 0;
@@ -531,10 +543,12 @@ sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint3
         uint32_t npx = h.dev.width * h.dev.height;
         o_lam[i] = n_lam;
         o_maj[i] = n_maj;
+        o_shl[i] = n_shl;
         o_xyz[i] = n_xyz;
         o_film[i] = n_film;
         n_lam += h.lam.size();
         n_maj += h.maj.size();
+        n_shl += h.shell_d.size();
         n_xyz += (size_t)npx * 3;
         if (items[i].film) n_film += (size_t)npx * h.nlam;
         sky::DevJob &j = ctx->host_jobs[k];
@@ -546,11 +560,13 @@ sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint3
         n_acc += sky::job_accum_bytes(j) / sizeof(float2);
         max_npx = npx > max_npx ? npx : max_npx;
         max_maj = h.maj.size() > max_maj ? (uint32_t)h.maj.size() : max_maj;
+        max_layers = h.dev.L > max_layers ? h.dev.L : max_layers;
         samples += (uint64_t)npx * h.dev.spp * h.dev.nlam;
     }
     const uint32_t meta_words = 8 + 4 * n;
     HIP_TRY(ctx->b_lam.ensure(n_lam * sizeof(sky::DevLambda)), "alloc batch lambda tables", SKY_ERR_OOM);
     HIP_TRY(ctx->b_maj.ensure(n_maj * sizeof(float)), "alloc batch majorants", SKY_ERR_OOM);
+    HIP_TRY(ctx->b_shells.ensure(n_shl * sizeof(double)), "alloc batch shell tables", SKY_ERR_OOM);
     HIP_TRY(ctx->b_xyz.ensure(n_xyz * sizeof(float)), "alloc batch xyz", SKY_ERR_OOM);
     if (n_film) HIP_TRY(ctx->b_film.ensure(n_film * sizeof(float)), "alloc batch film", SKY_ERR_OOM);
     HIP_TRY(ctx->b_accum.ensure(n_acc * sizeof(float2)), "alloc batch partial sums", SKY_ERR_OOM);
@@ -559,15 +575,18 @@ sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint3
     HIP_TRY(ctx->diag.ensure(64), "alloc diagnostics", SKY_ERR_OOM);
     std::vector<sky::DevLambda> all_lam(n_lam);
     std::vector<float> all_maj(n_maj);
+    std::vector<double> all_shl(n_shl);
     uint32_t *meta = (uint32_t *)ctx->b_meta.p;
     for (uint32_t k = 0; k < n; ++k) {
         uint32_t i = order[k];
         sky::HostScene &h = scenes[i];
         std::copy(h.lam.begin(), h.lam.end(), all_lam.begin() + o_lam[i]);
         std::copy(h.maj.begin(), h.maj.end(), all_maj.begin() + o_maj[i]);
+        std::copy(h.shell_d.begin(), h.shell_d.end(), all_shl.begin() + o_shl[i]);
         sky::DevJob &j = ctx->host_jobs[k];
         j.s.lam = (const sky::DevLambda *)ctx->b_lam.p + o_lam[i];
         j.s.maj = (const float *)ctx->b_maj.p + o_maj[i];
+        j.s.shell_d = (const double *)ctx->b_shells.p + o_shl[i];
         j.s.xyz = (float *)ctx->b_xyz.p + o_xyz[i];
         j.s.film = items[i].film ? (float *)ctx->b_film.p + o_film[i] : nullptr;
         j.s.film_m2 = nullptr;
@@ -580,10 +599,12 @@ sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint3
                            stream), "upload batch lambda tables", SKY_ERR_COPY);
     HIP_TRY(hipMemcpyAsync(ctx->b_maj.p, all_maj.data(), n_maj * sizeof(float), hipMemcpyHostToDevice, stream),
             "upload batch majorants", SKY_ERR_COPY);
+    HIP_TRY(hipMemcpyAsync(ctx->b_shells.p, all_shl.data(), n_shl * sizeof(double), hipMemcpyHostToDevice, stream),
+            "upload batch shell tables", SKY_ERR_COPY);
     HIP_TRY(hipMemcpyAsync(ctx->b_jobs.p, ctx->host_jobs.data(), n * sizeof(sky::DevJob), hipMemcpyHostToDevice,
                            stream), "upload batch job table", SKY_ERR_COPY);
     HIP_TRY(sky::launch_zero((uint32_t *)ctx->diag.p, 16, stream), "clear diagnostics", SKY_ERR_SYNC);
-    sky::WavePlan plan = sky::plan_launch(ctx->tuning, ctx->num_cus, total_items, max_maj);
+    sky::WavePlan plan = sky::plan_launch(ctx->tuning, ctx->num_cus, total_items, max_maj, max_layers);
 
     // The graph holds only kernel nodes (zero, integrate, finalize); the
     // tables it reads are uploaded above on every call.

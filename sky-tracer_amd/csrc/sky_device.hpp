@@ -42,8 +42,11 @@ struct DevLambda {
 // Everything one render needs on the device.
 struct DevScene {
     double R, top, dz, delta_top, obs_y;
-    double inv_dz;     // 1 / dz
-    double kd1, kd2;   // shell k radius r_k: r_k^2 - R^2 = k kd1 + k^2 kd2 (2 R dz, dz^2)
+    double shell_tol;  // r^2 - R^2 within which an inward origin counts as on a shell boundary
+    // Shell k's lower boundary h_k (k = 0..L, h_L = top): uniform, or with
+    // thicknesses growing by a factor g.  First guess of the shell of
+    // altitude h: floor(h c2) (uniform, c1 = 0) or floor(log2(1 + h c1) c2).
+    float shell_c1, shell_c2;
     float Rf, R2f, topf, eps;
     float sun[3];
     float omc, cone_pdf;  // 1 - cos(sun radius) (= 2 sin^2(r/2)), 1/(2 pi omc)
@@ -58,6 +61,7 @@ struct DevScene {
     uint32_t dls, projection, max_depth, rr_start, watchdog_limit, max_segments, flags;
     const DevLambda *lam;
     const float *maj;  // [lambda * L + layer]
+    const double *shell_d;  // [L + 1]: r_k^2 - R^2 = h_k (2R + h_k)
     float *film;       // [pixel * nlam + lambda] (may be null)
     float *film_m2;    // same layout (may be null)
     float *xyz;        // [pixel * 3 + c] (may be null)
@@ -214,6 +218,27 @@ __device__ __forceinline__ f3 frame_dir(f3 n, float mu, float st, float phi_rev)
 }
 
 // ------------------------------------------------------------- rays
+typedef __attribute__((address_space(3))) const double *LdsDoublePtr;
+typedef __attribute__((address_space(1))) const double *GlobalDoublePtr;
+
+// The shell table r_k^2 - R^2 (k = 0..L) read from LDS or global memory; the
+// typed pointers keep each read a plain LDS or global load.
+struct ShellTab {
+    LdsDoublePtr lds;
+    GlobalDoublePtr glob;
+    bool use_lds;
+    __device__ __forceinline__ double operator[](int k) const { return use_lds ? lds[k] : glob[k]; }
+};
+
+__device__ __forceinline__ ShellTab global_shells(const DevScene &s)
+{
+    ShellTab t;
+    t.lds = (LdsDoublePtr) nullptr;
+    t.glob = (GlobalDoublePtr)s.shell_d;
+    t.use_lds = false;
+    return t;
+}
+
 struct Ray {
     double ox, oy, oz;  // origin (km), FP64
     double b, q;        // o.d and |o|^2 - (o.d)^2 - R^2
@@ -227,9 +252,11 @@ struct Ray {
 
 // Ray set-up: next_boundary (sky_trace.wgsl:233-250) from the impact
 // parameter -- b^2 - c of wgsl:212-215 is D - q with D = r_shell^2 - R^2 --
-// and the initial shell of the walk.  d is an FP32-normalised direction.
-__device__ __forceinline__ void ray_setup(const DevScene &s, double ox, double oy, double oz, f3 d,
-                                          Ray &r)
+// and the initial shell of the walk, found by comparing |o|^2 - R^2 with the
+// shell table (layer_for_altitude, wgsl:252-258, generalised to non-uniform
+// shells).  d is an FP32-normalised direction.
+__device__ __forceinline__ void ray_setup(const DevScene &s, const ShellTab &tab, double ox, double oy,
+                                          double oz, f3 d, Ray &r)
 {
     double dx = d.x, dy = d.y, dz = d.z;
     double inv = inv_norm64(dx, dy, dz);
@@ -262,29 +289,26 @@ __device__ __forceinline__ void ray_setup(const DevScene &s, double ox, double o
         r.tmax = (float)t_exit;
         r.kind = 1;
     }
-    double h0 = sqrt64(oo) - s.R;
-    int k = (int)floor(h0 * s.inv_dz);
+    const double w0 = oo - s.R * s.R;
+    const int L = (int)s.L;
+    const float wf = (float)w0;
+    const float h0 = f_div(wf, f_sqrt(fmaxf(s.R2f + wf, 0.0f)) + s.Rf);
+    float kg = s.shell_c1 > 0.0f ? f_log2(fmaxf(1.0f + h0 * s.shell_c1, 1.0f)) * s.shell_c2 : h0 * s.shell_c2;
+    int k = (int)fminf(fmaxf(kg, 0.0f), (float)(L - 1));
+    while (k > 0 && tab[k] > w0) --k;
+    while (k < L - 1 && tab[k + 1] <= w0) ++k;
     r.outward = b >= 0.0;
-    if (!r.outward && k > 0 && h0 - k * s.dz <= 1e-9) k -= 1;
-    k = k < 0 ? 0 : k;
-    k = k > (int)s.L - 1 ? (int)s.L - 1 : k;
+    if (!r.outward && k > 0 && w0 - tab[k] <= s.shell_tol) k -= 1;
     r.layer = k;
-}
-
-// r_k^2 - R^2 - q for shell k.
-__device__ __forceinline__ double shell_disc(const DevScene &s, int k, double q)
-{
-    double fk = (double)k;
-    return fk * (s.kd1 + fk * s.kd2) - q;
 }
 
 // Next crossing of the majorant-shell walk (the segments of
 // sky_trace.wgsl:284-294): D_k - q in FP64, its root in FP32.
-__device__ __forceinline__ float next_crossing(const DevScene &s, double q, float bf, int k, int out,
-                                               int &kn, int &on)
+__device__ __forceinline__ float next_crossing(const DevScene &s, const ShellTab &tab, double q, float bf, int k,
+                                               int out, int &kn, int &on)
 {
     if (!out) {
-        double disc = shell_disc(s, k, q);
+        double disc = tab[k] - q;
         if (disc > 0.0) {
             on = 0;
             if (k == 0) {
@@ -301,14 +325,14 @@ __device__ __forceinline__ float next_crossing(const DevScene &s, double q, floa
         return __builtin_inff();
     }
     kn = k + 1;
-    double disc = shell_disc(s, k + 1, q);
+    double disc = tab[k + 1] - q;
     return f_sqrt((float)(disc > 0.0 ? disc : 0.0)) - bf;
 }
 
-__device__ __forceinline__ float next_crossing(const DevScene &s, const Ray &r, int k, int out, int &kn,
-                                               int &on)
+__device__ __forceinline__ float next_crossing(const DevScene &s, const ShellTab &tab, const Ray &r, int k,
+                                               int out, int &kn, int &on)
 {
-    return next_crossing(s, r.q, r.bf, k, out, kn, on);
+    return next_crossing(s, tab, r.q, r.bf, k, out, kn, on);
 }
 
 // Altitude at distance t: (q + (b + t)^2) / (sqrt(R^2 + q + (b + t)^2) + R).

@@ -18,6 +18,7 @@ struct HostScene {
     DevScene dev{};
     std::vector<DevLambda> lam;
     std::vector<float> maj;
+    std::vector<double> shell_d;  // [L + 1] r_k^2 - R^2 of the shell boundaries
     uint32_t nlam = 0;
 };
 
@@ -28,8 +29,9 @@ sky_status build_scene(const sky_planet *planet, const sky_atmosphere *atm,
 
 void fill_wavelength_rows(const sky_atmosphere *atm, const sky_wavelengths *wl,
                           std::vector<DevLambda> &rows);
+double shell_height(double top, uint32_t layers, double growth, uint32_t k);
 void fill_majorants(const sky_planet *planet, const sky_atmosphere *atm,
-                    const sky_wavelengths *wl, uint32_t layers, double margin,
+                    const sky_wavelengths *wl, uint32_t layers, double margin, double growth,
                     std::vector<float> &out);
 
 struct LaunchShape {
@@ -58,6 +60,7 @@ struct PoolArgs {
     unsigned long long *t_start;
     uint32_t event_threshold, track_steps;
     uint32_t maj_lds;  // floats per wave of the LDS majorant copy (0: read the grids from global memory)
+    uint32_t shell_lds;  // doubles per wave of the LDS shell table
 };
 
 // Inputs of the adjoint replay (sky_render_adjoint).
@@ -73,14 +76,17 @@ struct AdjArgs {
 struct WavePlan {
     uint32_t n_items = 0, grid = 0, block = 256, event_threshold = 32, track_steps = 8, occupancy = 0;
     uint32_t maj_lds = 0;   // floats per wave for the majorant grid in LDS (0: global)
+    uint32_t shell_lds = 0; // doubles per wave for the shell table in LDS (0 with maj_lds 0)
     uint32_t lds_bytes = 0; // dynamic LDS per block
 };
 
 void plan_job(const DevScene &s, const sky_tuning &t, int num_cus, uint32_t min_items, DevJob &j);
 size_t job_accum_bytes(const DevJob &j);
 uint32_t launch_min_items(const sky_tuning &t, int num_cus);
-// max_maj: the largest nlam x majorant_layers over the launch's jobs.
-WavePlan plan_launch(const sky_tuning &t, int num_cus, uint32_t total_items, uint32_t max_maj);
+// max_maj: the largest nlam x majorant_layers over the launch's jobs;
+// max_layers: the largest majorant_layers.
+WavePlan plan_launch(const sky_tuning &t, int num_cus, uint32_t total_items, uint32_t max_maj,
+                     uint32_t max_layers);
 hipError_t launch_pool(const DevJob *d_jobs, uint32_t n_jobs, uint32_t max_npx, const WavePlan &p,
                        uint32_t *meta, uint32_t meta_words, hipStream_t stream);
 hipError_t launch_zero(uint32_t *p, uint32_t n, hipStream_t stream);

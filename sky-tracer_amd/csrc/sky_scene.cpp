@@ -92,17 +92,30 @@ void fill_wavelength_rows(const sky_atmosphere *a, const sky_wavelengths *wl,
 // (medium.rs:77, sky_trace.wgsl:260).  Exact for these profiles: the
 // exponentials peak at the shell's lower edge, the tent at the point of the
 // shell nearest its centre.
+// Lower boundary of shell k (k = 0..L, h_L = top): the reference's uniform
+// shells (medium.rs:63-83) for growth 1, else thicknesses growing by the
+// factor g from the ground up: h_k = top (g^k - 1) / (g^L - 1).
+double shell_height(double top, uint32_t layers, double growth, uint32_t k)
+{
+    if (k == 0) return 0.0;
+    if (k >= layers) return top;
+    if (!(growth > 1.0)) return k * (top / layers);
+    return top * (std::pow(growth, (double)k) - 1.0) / (std::pow(growth, (double)layers) - 1.0);
+}
+
+// Majorant grid (medium.rs:63-83): margin x the exact maximum of sigma_t
+// over each shell (both exponentials peak at the shell's lower edge, the
+// ozone tent at the point nearest its centre), floored at 1e-8.
 void fill_majorants(const sky_planet *planet, const sky_atmosphere *a, const sky_wavelengths *wl,
-                    uint32_t layers, double margin, std::vector<float> &out)
+                    uint32_t layers, double margin, double growth, std::vector<float> &out)
 {
     double top = planet->atmosphere_radius_km - planet->ground_radius_km;
-    double dz = top / layers;
     out.resize((size_t)wl->count * layers);
     for (size_t b = 0; b < wl->count; ++b) {
         double c[4];
         coeffs_d(a, wl->center_nm[b], c);
         for (uint32_t k = 0; k < layers; ++k) {
-            double lo = k * dz, hi = (k + 1) * dz;
+            double lo = shell_height(top, layers, growth, k), hi = shell_height(top, layers, growth, k + 1);
             double d = 0.0;
             if (a->ozone_center_km < lo) d = lo - a->ozone_center_km;
             if (a->ozone_center_km > hi) d = a->ozone_center_km - hi;
@@ -187,6 +200,11 @@ sky_status build_scene(const sky_planet *planet, const sky_atmosphere *a, const 
         err = "unknown output projection";
         return SKY_ERR_INVALID_CONFIG;
     }
+    const double growth = cfg->shell_growth == 0.0 ? 1.0 : cfg->shell_growth;
+    if (!(growth >= 1.0 && growth <= 16.0) || cfg->majorant_layers * std::log2(growth) > 1000.0) {
+        err = "shell growth must lie in [1, 16] (0 means 1)";
+        return SKY_ERR_INVALID_CONFIG;
+    }
 
     out.nlam = wl->count;
     fill_wavelength_rows(a, wl, out.lam);
@@ -200,7 +218,7 @@ sky_status build_scene(const sky_planet *planet, const sky_atmosphere *a, const 
             }
         }
     } else {
-        fill_majorants(planet, a, wl, cfg->majorant_layers, cfg->majorant_margin, out.maj);
+        fill_majorants(planet, a, wl, cfg->majorant_layers, cfg->majorant_margin, growth, out.maj);
     }
 
     DevScene &s = out.dev;
@@ -209,9 +227,19 @@ sky_status build_scene(const sky_planet *planet, const sky_atmosphere *a, const 
     s.top = planet->atmosphere_radius_km - planet->ground_radius_km;
     s.L = cfg->majorant_layers;
     s.dz = s.top / s.L;
-    s.inv_dz = 1.0 / s.dz;
-    s.kd1 = 2.0 * s.R * s.dz;
-    s.kd2 = s.dz * s.dz;
+    out.shell_d.resize(s.L + 1);
+    for (uint32_t k = 0; k <= s.L; ++k) {
+        double h = shell_height(s.top, s.L, growth, k);
+        out.shell_d[k] = h * (2.0 * s.R + h);
+    }
+    s.shell_tol = 2e-9 * s.R;  // 1e-9 km of height
+    if (growth > 1.0) {
+        s.shell_c1 = (float)((std::pow(growth, (double)s.L) - 1.0) / s.top);
+        s.shell_c2 = (float)(1.0 / std::log2(growth));
+    } else {
+        s.shell_c1 = 0.0f;
+        s.shell_c2 = (float)(s.L / s.top);
+    }
     s.delta_top = (planet->atmosphere_radius_km - s.R) * (planet->atmosphere_radius_km + s.R);
     s.obs_y = s.R + std::fmax(cfg->observer_altitude_km, 0.0);
     s.Rf = (float)s.R;

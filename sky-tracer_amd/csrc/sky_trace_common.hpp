@@ -37,7 +37,7 @@ __device__ __forceinline__ bool track(const DevScene &s, const DevLambda &l, con
         }
         guard++;
         int kn, on;
-        float tc = next_crossing(s, r, k, out, kn, on);
+        float tc = next_crossing(s, global_shells(s), r, k, out, kn, on);
         float seg_end = fminf(tc, r.tmax);
         seg_end = fmaxf(seg_end, t);
         float maj = maj_row[k];
@@ -122,7 +122,7 @@ __device__ float trace_path(const DevScene &s, uint32_t x, uint32_t y, uint32_t 
             return L;
         }
         Ray ray;
-        ray_setup(s, ox, oy, oz, dir, ray);
+        ray_setup(s, global_shells(s), ox, oy, oz, dir, ray);
         if (!ray.hit) break;
         float th, Tdummy;
         Coeffs co;
@@ -145,7 +145,7 @@ __device__ float trace_path(const DevScene &s, uint32_t x, uint32_t y, uint32_t 
                 float a1 = rng_next(rng);
                 f3 ld = sun_cone_dir(s, a0, a1);
                 Ray sr;
-                ray_setup(s, px + (double)ld.x * kRayEpsilonKm, py + (double)ld.y * kRayEpsilonKm,
+                ray_setup(s, global_shells(s), px + (double)ld.x * kRayEpsilonKm, py + (double)ld.y * kRayEpsilonKm,
                           pz + (double)ld.z * kRayEpsilonKm, ld, sr);
                 if (sr.hit && sr.kind == 1) {
                     float T = transmittance(s, l, maj_row, sr, rng, wd);
@@ -196,7 +196,7 @@ __device__ float trace_path(const DevScene &s, uint32_t x, uint32_t y, uint32_t 
                 float cs = fmaxf(dot3(nrm, ld), 0.0f);
                 if (cs > 0.0f) {
                     Ray sr;
-                    ray_setup(s, px + (double)ld.x * kRayEpsilonKm, py + (double)ld.y * kRayEpsilonKm,
+                    ray_setup(s, global_shells(s), px + (double)ld.x * kRayEpsilonKm, py + (double)ld.y * kRayEpsilonKm,
                               pz + (double)ld.z * kRayEpsilonKm, ld, sr);
                     if (sr.hit && sr.kind == 1) {
                         float T = transmittance(s, l, maj_row, sr, rng, wd);

@@ -188,7 +188,7 @@ __device__ __forceinline__ void add_dlog_phase(const DevScene &s, const AdjArgs 
 // removes a global-load latency from every shell crossing.
 constexpr uint32_t kMajLdsMax = 2048;
 // Per-wave LDS layout (floats): sums [256], sums of squares [256], adjoint
-// gradient [16], majorant grid [maj_lds].
+// gradient [16], majorant grid [maj_lds], shell table [shell_lds doubles].
 constexpr uint32_t kLdsFixed = 2 * kSlotsPerWave + 16;
 
 typedef __attribute__((address_space(3))) const float *LdsFloatPtr;
@@ -201,6 +201,7 @@ struct MajSrc {
     LdsFloatPtr lds;      // wave's LDS copy (when use_lds)
     GlobalFloatPtr glob;  // the job's grid in global memory
     bool use_lds;
+    ShellTab shells;      // the job's shell table, LDS copy with the grid
 };
 
 __device__ __forceinline__ float majorant(const MajSrc &m, uint32_t L, uint32_t band, int k)
@@ -212,7 +213,7 @@ __device__ __forceinline__ float majorant(const MajSrc &m, uint32_t L, uint32_t 
 
 __device__ __forceinline__ void seg_begin(const DevScene &s, const MajSrc &m, LaneCtx &c)
 {
-    float tc = next_crossing(s, c.q, c.bf, c.k, c.out, c.kn, c.on);
+    float tc = next_crossing(s, m.shells, c.q, c.bf, c.k, c.out, c.kn, c.on);
     c.seg_end = fmaxf(fminf(tc, c.tmax), c.t);
     c.maj = majorant(m, s.L, c.band, c.k);
 }
@@ -223,7 +224,7 @@ __device__ __forceinline__ bool ray_begin(const DevScene &s, const MajSrc &m, La
                                           double oy, double oz, f3 d, int mode)
 {
     Ray r;
-    ray_setup(s, ox, oy, oz, d, r);
+    ray_setup(s, m.shells, ox, oy, oz, d, r);
     c.d = d;
     c.bf = r.bf;
     c.qf = r.qf;
@@ -691,11 +692,12 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
     uint32_t loaded = 0xFFFFFFFFu;  // job whose majorants are in the wave's LDS copy
     const uint32_t wid = threadIdx.x >> 6;
     const uint32_t lane = lane_id();
-    float *wave_lds = dyn_lds + wid * (kLdsFixed + a.maj_lds);
+    float *wave_lds = dyn_lds + wid * (kLdsFixed + a.maj_lds + 2 * a.shell_lds);
     float *acc_sum = wave_lds;
     float *acc_sq = wave_lds + kSlotsPerWave;
     float *acc_grad = wave_lds + 2 * kSlotsPerWave;
     float *lds_maj = wave_lds + kLdsFixed;
+    double *lds_shell = (double *)(lds_maj + a.maj_lds);
     bool wd = false;
     uint32_t *diag = nullptr;
     LaneCtx c;
@@ -733,10 +735,14 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
         msrc.lds = (LdsFloatPtr)lds_maj;
         msrc.glob = (GlobalFloatPtr)s.maj;
         const uint32_t nmaj = s.nlam * s.L;
-        msrc.use_lds = nmaj <= a.maj_lds;
+        msrc.use_lds = nmaj <= a.maj_lds && s.L + 1 <= a.shell_lds;
+        msrc.shells.lds = (LdsDoublePtr)lds_shell;
+        msrc.shells.glob = (GlobalDoublePtr)s.shell_d;
+        msrc.shells.use_lds = msrc.use_lds;
         if (msrc.use_lds) {
             if (loaded != jid) {
                 for (uint32_t i = lane; i < nmaj; i += 64) lds_maj[i] = s.maj[i];
+                for (uint32_t i = lane; i <= s.L; i += 64) lds_shell[i] = s.shell_d[i];
                 loaded = jid;
             }
         }
@@ -884,11 +890,13 @@ size_t job_accum_bytes(const DevJob &j)
     return (size_t)j.chunks * j.s.width * j.s.height * j.s.nlam * sizeof(float2);
 }
 
-WavePlan plan_launch(const sky_tuning &t, int num_cus, uint32_t total_items, uint32_t max_maj)
+WavePlan plan_launch(const sky_tuning &t, int num_cus, uint32_t total_items, uint32_t max_maj,
+                     uint32_t max_layers)
 {
     WavePlan p;
     p.maj_lds = max_maj <= kMajLdsMax ? (max_maj + 63u) & ~63u : 0u;
-    p.lds_bytes = kWavesPerBlock * (kLdsFixed + p.maj_lds) * (uint32_t)sizeof(float);
+    p.shell_lds = p.maj_lds ? ((max_layers + 1u + 7u) & ~7u) : 0u;
+    p.lds_bytes = kWavesPerBlock * (kLdsFixed + p.maj_lds + 2 * p.shell_lds) * (uint32_t)sizeof(float);
     uint32_t wpc = t.waves_per_cu ? t.waves_per_cu : 16u;
     uint64_t waves = (uint64_t)(num_cus > 0 ? num_cus : 256) * wpc;
     uint64_t w = waves < total_items ? waves : total_items;
@@ -922,6 +930,7 @@ hipError_t launch_pool(const DevJob *d_jobs, uint32_t n_jobs, uint32_t max_npx, 
     a.event_threshold = p.event_threshold;
     a.track_steps = p.track_steps;
     a.maj_lds = p.maj_lds;
+    a.shell_lds = p.shell_lds;
     switch (p.occupancy) {
     case 4: hipLaunchKernelGGL(k_wavepool_o4, dim3(p.grid), dim3(p.block), p.lds_bytes, stream, a); break;
     case 5: hipLaunchKernelGGL(k_wavepool_o5, dim3(p.grid), dim3(p.block), p.lds_bytes, stream, a); break;
@@ -948,6 +957,7 @@ hipError_t launch_adjoint(const DevJob *d_jobs, const WavePlan &p, uint32_t *met
     a.event_threshold = p.event_threshold;
     a.track_steps = p.track_steps;
     a.maj_lds = p.maj_lds;
+    a.shell_lds = p.shell_lds;
     hipLaunchKernelGGL(k_wavepool_adjoint, dim3(p.grid), dim3(p.block), p.lds_bytes, stream, a, A);
     return hipGetLastError();
 }

@@ -83,6 +83,7 @@ class RenderConfigC(C.Structure):
         ("flags", C.c_uint32),
         ("majorant_margin", C.c_double),
         ("majorant_table", C.POINTER(C.c_float)),
+        ("shell_growth", C.c_double),
     ]
 
 
@@ -163,7 +164,7 @@ def lib() -> C.CDLL:
         "sky_default_atmosphere": ([A], None),
         "sky_default_config": ([R], None),
         "sky_wavelength_table": ([A, W, fp, C.c_uint32], C.c_int),
-        "sky_majorant_grid": ([P, A, W, C.c_uint32, C.c_double, fp, C.c_uint32], C.c_int),
+        "sky_majorant_grid": ([P, A, W, C.c_uint32, C.c_double, C.c_double, fp, C.c_uint32], C.c_int),
         "sky_validate_config": ([P, A, W, R], C.c_int),
         "sky_render": ([ctxp, P, A, W, R, fp, fp, fp, C.POINTER(RenderStats)], C.c_int),
         "sky_render_device": ([ctxp, P, A, W, R, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,

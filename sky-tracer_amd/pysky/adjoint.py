@@ -32,7 +32,7 @@ def with_params(atm: AtmosphereParams, theta) -> AtmosphereParams:
 def fixed_majorant(scene: SceneData, config: RenderConfig, margin: float = 1.25) -> np.ndarray:
     """A majorant grid for the scene's current parameters with extra margin,
     to hold fixed while the parameters move (FD, adjoint, fitting)."""
-    return majorant_grid(scene, config.majorant_layers, margin)
+    return majorant_grid(scene, config.majorant_layers, margin, config.shell_growth)
 
 
 def render_adjoint(scene: SceneData, config: RenderConfig, dl_dxyz: np.ndarray,

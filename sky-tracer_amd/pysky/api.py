@@ -118,6 +118,9 @@ class RenderConfig:
     simple_kernel: bool = False
     # optional fixed majorant grid [bands, layers] (km^-1)
     majorant_table: Optional[np.ndarray] = None
+    # Majorant shell thickness growth from the ground up (1: uniform shells,
+    # the reference's); any value samples the same free-flight law.
+    shell_growth: float = 1.0
 
     @property
     def flags(self) -> int:
@@ -250,6 +253,7 @@ def config_c(scene: SceneData, cfg: RenderConfig) -> L.RenderConfigC:
     c.majorant_layers = cfg.majorant_layers
     c.flags = cfg.flags
     c.majorant_margin = cfg.majorant_margin
+    c.shell_growth = cfg.shell_growth
     if cfg.majorant_table is not None:
         t = np.ascontiguousarray(cfg.majorant_table, dtype=np.float32).ravel()
         if t.size != scene.band_count * cfg.majorant_layers:
@@ -395,11 +399,12 @@ def wavelength_table(scene: SceneData) -> np.ndarray:
     return out.reshape(nb, 8)
 
 
-def majorant_grid(scene: SceneData, layers: int = 64, margin: float = 1.05) -> np.ndarray:
-    """MajorantGrid values (medium.rs:63-83), [band, layer] in km^-1 (host)."""
+def majorant_grid(scene: SceneData, layers: int = 64, margin: float = 1.05, shell_growth: float = 1.0) -> np.ndarray:
+    """MajorantGrid values (medium.rs:63-83), [band, layer] in km^-1 (host),
+    over uniform shells or shells growing by `shell_growth`."""
     nb = scene.band_count
     out = np.zeros(nb * layers, dtype=np.float32)
     planet, atm, wl = scene.planet_c(), scene.atmosphere.to_c(), scene.wavelengths_c()
-    check(L.lib().sky_majorant_grid(C.byref(planet), C.byref(atm), C.byref(wl), layers, margin,
+    check(L.lib().sky_majorant_grid(C.byref(planet), C.byref(atm), C.byref(wl), layers, margin, shell_growth,
                                     _f32p(out), out.size))
     return out.reshape(nb, layers)

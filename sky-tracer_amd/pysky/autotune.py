@@ -1,8 +1,9 @@
 """Autotuner of the wave-pool integrator's launch parameters.
 
 Knobs (none changes the estimator's expectation; each is checked by the
-parity tests): majorant shells L (delta tracking samples the exact
-free-flight law for any majorant), event-batching threshold, tracking
+parity tests): majorant shells L and their thickness growth g (delta
+tracking samples the exact free-flight law for any majorant),
+event-batching threshold, tracking
 micro-steps per handler pass, register budget (occupancy variant),
 persistent waves per CU, pixels per work item (lambda packing: pixels x
 wavelengths per wave).  Coordinate descent over short renders of the actual
@@ -22,9 +23,10 @@ from .api import Context, RenderConfig, SceneData, render
 CACHE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_cache.json")
 
 DEFAULT = dict(majorant_layers=16, event_threshold=32, track_steps=8, occupancy=0, waves_per_cu=16,
-               pixels_per_item=0)
+               pixels_per_item=0, shell_growth=1.0)
 SPACE = {
-    "majorant_layers": [4, 5, 6, 7, 8, 12, 16, 24, 32, 64],
+    "majorant_layers": [2, 3, 4, 5, 6, 7, 8, 12, 16, 24, 32, 64],
+    "shell_growth": [1.0, 1.5, 2.0, 2.5, 3.0, 4.0, 5.0],
     "event_threshold": [8, 16, 24, 32, 40, 48, 56],
     "track_steps": [1, 2, 4, 8, 16, 32],
     "occupancy": [0, 4, 5, 6, 8],
@@ -58,7 +60,8 @@ def apply(ctx: Context, config: RenderConfig, params: dict) -> RenderConfig:
     ctx.set_tuning(waves_per_cu=params["waves_per_cu"], pixels_per_item=params["pixels_per_item"],
                    event_threshold=params["event_threshold"], track_steps=params["track_steps"],
                    occupancy=params["occupancy"])
-    return dataclasses.replace(config, majorant_layers=params["majorant_layers"], majorant_table=None)
+    return dataclasses.replace(config, majorant_layers=params["majorant_layers"],
+                               shell_growth=params.get("shell_growth", 1.0), majorant_table=None)
 
 
 def _time(ctx: Context, scene: SceneData, config: RenderConfig, params: dict, reps: int = 2) -> float:

@@ -1,6 +1,7 @@
 """Freezes the algorithmic-flop model used by bench.py's roofline.
 
-F_unit(config, mode) = sum_e  n_e(config, mode) * c_e, where n_e is the mean
+F_unit(config, mode) = sum_e  n_e(config, mode) * c_e (per majorant shell count
+L and shell growth g), where n_e is the mean
 count per path x lambda sample of event e measured by the CPU oracle at the
 configuration's seed on its subsampled pixel grid, and c_e the fixed FP32 cost
 of the event's arithmetic (FMA = 2, add/mul/cmp/min/max/cvt = 1, each
@@ -48,12 +49,17 @@ MODES = {
 }
 
 
-def measure(workload, mode, layers):
+def key(workload, mode, layers, growth=1.0):
+    k = f"{workload}:{mode}:L{layers}"
+    return k if growth == 1.0 else f"{k}:g{growth:g}"
+
+
+def measure(workload, mode, layers, growth=1.0):
     w, h, nl, spp, depth, sun, alb, aer, step = WORKLOADS[workload]
     p = O.Params(width=w, height=h, spp=spp, seed=1, sun_elevation_deg=sun, max_depth=depth,
                  ground_albedo=alb, aerosol_sca_550_m_inv=3.996e-6 * aer,
                  aerosol_abs_550_m_inv=4.4e-6 * aer, lambda_nm=list(O.midpoint_grid(nl)),
-                 flags=MODES[mode], majorant_layers=layers)
+                 flags=MODES[mode], majorant_layers=layers, shell_growth=growth)
     xs = np.arange(step // 2, w, step)
     ys = np.arange(step // 2, h, step)
     pixels = (ys[:, None] * w + xs[None, :]).ravel()
@@ -73,12 +79,18 @@ def measure(workload, mode, layers):
 def main():
     out = {"costs": COSTS, "int_ops_per_rng_draw": INT_OPS_PER_DRAW, "configs": {}}
     for wl in ("c1", "c2", "c3"):
-        for mode in ("reference", "carried", "carried_is"):
-            layer_set = (4, 5, 6, 7, 8, 12, 16, 24, 32, 64) if mode == "carried_is" else (16, 64)
-            for layers in layer_set:
-                key = f"{wl}:{mode}:L{layers}"
-                out["configs"][key] = measure(wl, mode, layers)
-                print(key, round(out["configs"][key]["flops_per_sample"], 1), flush=True)
+        for mode in ("reference", "carried", "carried_is", "carried_is_ratio"):
+            if mode in ("carried_is", "carried_is_ratio"):
+                combos = [(L, 1.0) for L in (4, 5, 6, 7, 8, 12, 16, 24, 32, 64)]
+                combos += [(L, g) for L in (2, 3, 4, 5, 6, 7, 8) for g in (1.5, 2.0, 2.5, 3.0, 4.0, 5.0)]
+                if mode == "carried_is_ratio":
+                    combos = [(L, g) for L, g in combos if L <= 8]
+            else:
+                combos = [(16, 1.0), (64, 1.0)]
+            for layers, g in combos:
+                k = key(wl, mode, layers, g)
+                out["configs"][k] = measure(wl, mode, layers, g)
+                print(k, round(out["configs"][k]["flops_per_sample"], 1), flush=True)
     dst = os.path.join(ROOT, "sky-tracer_amd", "pysky", "flop_model.json")
     with open(dst, "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)

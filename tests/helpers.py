@@ -48,7 +48,7 @@ def case(width=128, height=64, n_lambda=16, spp=64, sun=10.0, max_depth=8, seed=
                   flags=oracle_flags(**fl),
                   **{k: v for k, v in extra.items() if k in ("direct_light_samples", "rr_start_depth",
                                                              "watchdog_limit", "max_segments",
-                                                             "observer_altitude_km")})
+                                                             "observer_altitude_km", "shell_growth")})
     return scene, cfg, op
 
 

@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_sizes_match_header_layout():
     # the Python mirrors must have the C layout (no padding surprises)
-    assert C.sizeof(L.RenderConfigC) == 4 * 4 + 8 + 3 * 8 + 8 * 4 + 8 + 8
+    assert C.sizeof(L.RenderConfigC) == 4 * 4 + 8 + 3 * 8 + 8 * 4 + 8 + 8 + 8
     assert C.sizeof(L.Atmosphere) == 15 * 8
     assert C.sizeof(L.Tuning) == 5 * 4
 
@@ -59,16 +59,17 @@ def test_wavelength_table_matches_oracle(n):
     np.testing.assert_allclose(got, ref, rtol=2e-7)
 
 
-@pytest.mark.parametrize("layers", [8, 16, 64, 200])
-def test_majorant_grid_matches_oracle(layers):
+@pytest.mark.parametrize("layers,growth", [(8, 1.0), (16, 1.0), (64, 1.0), (200, 1.0), (3, 3.0), (6, 1.7)])
+def test_majorant_grid_matches_oracle(layers, growth):
     scene = pysky.SceneData(wavelengths_nm=list(pysky.midpoint_grid(16)))
-    got = pysky.majorant_grid(scene, layers)
-    ref = O.majorant_grid(O.Params(majorant_layers=layers))
+    got = pysky.majorant_grid(scene, layers, shell_growth=growth)
+    ref = O.majorant_grid(O.Params(majorant_layers=layers, shell_growth=growth))
     np.testing.assert_allclose(got, ref, rtol=2e-7)
 
 
 @pytest.mark.parametrize("bad", [
     dict(width=0), dict(height=0), dict(spp=0), dict(majorant_layers=0), dict(majorant_layers=300),
+    dict(shell_growth=0.5), dict(shell_growth=20.0), dict(majorant_layers=256, shell_growth=16.0),
     dict(majorant_margin=0.9), dict(output_projection=2), dict(width=70000, height=70000),
 ])
 def test_invalid_config_is_reported_before_device(bad):

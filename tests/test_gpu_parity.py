@@ -41,6 +41,21 @@ def test_paths_pair_with_oracle(gpu_context, mode, simple):
     assert np.all(np.isfinite(film)) and np.all(film >= 0)
 
 
+@pytest.mark.parametrize("mode,layers,growth", [("carried_is", 3, 3.0), ("reference", 7, 1.6), ("carried", 5, 2.5)])
+@pytest.mark.parametrize("simple", [False, True])
+def test_growing_shells_pair_with_oracle(gpu_context, mode, layers, growth, simple):
+    """Non-uniform majorant shells (thickness growing by `growth`): the shell
+    table, the initial-shell search and the crossings follow the oracle's."""
+    scene, cfg, op = case(width=64, height=32, spp=1, mode=mode, simple_kernel=simple, max_depth=8,
+                          layers=layers, shell_growth=growth)
+    film = _gpu(scene, cfg, gpu_context).values()
+    ref, _, _, _, rc = O.render(op, "f32")
+    assert rc == 0
+    frac, bad = paired_stats(film, ref.ravel(), rtol=1e-3)
+    print(f"paired fraction {frac:.5f} ({len(bad)} of {film.size} differ)")
+    assert frac >= 0.995
+
+
 @pytest.mark.parametrize("mode", ["carried", "carried_is"])
 def test_sample_offset_pairs(gpu_context, mode):
     # a later sample index exercises other RNG keys (seed_hi ^ sample)

@@ -164,14 +164,26 @@ def test_medium_closed_form():
     assert O.coefficients(p, 0, -0.01).sum() == 0.0 and O.coefficients(p, 0, 100.01).sum() == 0.0
 
 
-def test_majorants_dominate_extinction():
+def test_shell_heights():
+    """Uniform shells are k top / L; growing shells are geometric with ratio
+    g between consecutive thicknesses and still span [0, top]."""
+    h = O.shell_heights(O.Params(majorant_layers=8))
+    np.testing.assert_allclose(h, np.arange(9) * 12.5, rtol=0, atol=1e-12)
+    h = O.shell_heights(O.Params(majorant_layers=5, shell_growth=2.5))
+    assert h[0] == 0.0 and h[-1] == pytest.approx(100.0)
+    np.testing.assert_allclose(np.diff(h)[1:] / np.diff(h)[:-1], 2.5, rtol=1e-12)
+
+
+@pytest.mark.parametrize("layers,growth", [(64, 1.0), (4, 3.0), (7, 1.6)])
+def test_majorants_dominate_extinction(layers, growth):
     """data.rs:253-276 on a dense grid: sigma_t <= majorant of its layer."""
     for kw in (dict(), dict(ozone_center_km=23.3), dict(aerosol_h_km=0.7, rayleigh_h_km=6.0)):
-        p = O.Params(lambda_nm=list(O.midpoint_grid(64)), **kw)
+        p = O.Params(lambda_nm=list(O.midpoint_grid(64)), majorant_layers=layers, shell_growth=growth, **kw)
         maj = O.majorant_grid(p)
         lam = O.lambda_table(p)
         h = np.linspace(0.0, 100.0, 24001)
-        layer = np.minimum((np.clip(h, 0, 100 - 1e-6) / 100.0 * 64).astype(int), 63)
+        edges = O.shell_heights(p)
+        layer = np.clip(np.searchsorted(edges, h, side="right") - 1, 0, layers - 1)
         ro = np.maximum(0.0, 1.0 - np.abs(h - p.ozone_center_km) / p.ozone_half_width_km)
         for b in range(64):
             st = lam[b, 0] * np.exp(-h / p.rayleigh_h_km) + (lam[b, 2] + lam[b, 3]) * np.exp(-h / p.aerosol_h_km) + lam[b, 1] * ro
