@@ -38,7 +38,7 @@ PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 (packed vector FMA); unpacked vector iss
 # a 256 MiB copy): profiles/k_wavepool_c2_1024x512x64_counters.md.  The launch writes one
 # [pixel][band] float2 partial per pixel group and reads only the scene tables, so the figure
 # does not depend on spp at the tuned work split.
-TRAFFIC_BYTES = {"c2": (2.0 * 105.9 + 270528.0) * 1024}
+TRAFFIC_BYTES = {"c2": (2.0 * 1853.5 + 278208.1) * 1024}
 
 
 def make_scene(w):
@@ -126,8 +126,8 @@ def bench_batch(args, ctx):
     from pysky.torch_ops import render_device
 
     batch = ragged_batch(n=256, seed=1, spp=args.spp or 256, max_depth=32)
-    layers = args.layers
-    batch = [(sc, dataclasses.replace(cfg, majorant_layers=layers)) for sc, cfg in batch]
+    layers, growth = args.layers, args.shell_growth
+    batch = [(sc, dataclasses.replace(cfg, majorant_layers=layers, shell_growth=growth)) for sc, cfg in batch]
     samples = sum(cfg.width * cfg.height * cfg.spp * sc.band_count for sc, cfg in batch)
     print(f"[bench] ragged batch: 256 renders, {samples:.3e} samples, layers={layers}", file=sys.stderr, flush=True)
     for i in range(args.warmup):
@@ -168,7 +168,7 @@ def bench_batch(args, ctx):
         "data": "synthetic (seeded ragged batch, analytic atmosphere)",
         "config": {"workload": "c4: 256 ragged renders (sun U[-6,90] deg, 64x32..512x256, 3..64 lambda in "
                                "400-700 nm, 256 spp, depth<=32), one launch, HIP graph",
-                   "estimator": "carried_is", "majorant_layers": layers},
+                   "estimator": "carried_is", "majorant_layers": layers, "shell_growth": growth},
         "kernel_ms": float(np.mean(kms)),
         "latency_p50_ms": float(np.mean(p50)), "latency_p99_ms": float(np.mean(p99)),
         "single_render_rate": single_rate, "single_render_kernel_ms_sum": single_kernel_ms,
@@ -191,7 +191,7 @@ def bench_adjoint(args, ctx):
     spp = args.spp or 256
     scene = pysky.SceneData(sun_elevation_deg=10.0, wavelengths_nm=list(pysky.midpoint_grid(32)))
     base = pysky.RenderConfig(width=256, height=128, spp=spp, seed=7, max_depth=8, rr_start_depth=4,
-                              majorant_layers=args.layers or 8, **MODES[args.mode])
+                              majorant_layers=args.layers, shell_growth=args.shell_growth, **MODES[args.mode])
     dev = torch.device("cuda:0")
     xyz = torch.zeros(128 * 256 * 3, dtype=torch.float32, device=dev)
     # loss = mean(log XYZ) over the image, so every pixel weighs by its relative
@@ -225,7 +225,8 @@ def bench_adjoint(args, ctx):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 ray set-up)",
         "data": "synthetic (analytic atmosphere)",
         "config": {"workload": f"c5: 256x128, 32 lambda, {spp} spp, depth<=8, sun 10 deg, 10-parameter adjoint",
-                   "estimator": args.mode, "majorant_layers": base.majorant_layers},
+                   "estimator": args.mode, "majorant_layers": base.majorant_layers,
+                   "shell_growth": base.shell_growth},
         "forward_ms": float(np.mean(f_ms)), "adjoint_ms": float(np.mean(a_ms)),
         "adjoint_over_forward": float(np.mean(a_ms) / np.mean(f_ms)),
         "grad_rel_rmse_at_spp": rel_rmse, "spp_for_1pct": spp_1pct,
@@ -247,7 +248,8 @@ def run_fit(args, ctx):
 
     scene = pysky.SceneData(sun_elevation_deg=10.0, wavelengths_nm=list(pysky.midpoint_grid(32)))
     cfg = pysky.RenderConfig(width=256, height=128, spp=args.spp or 256, seed=7, max_depth=8, rr_start_depth=4,
-                             carried_tau=True, phase_importance_sampling=True, majorant_layers=args.layers or 8)
+                             carried_tau=True, phase_importance_sampling=True, majorant_layers=args.layers,
+                             shell_growth=args.shell_growth)
     target = torch.zeros((128, 256, 3), dtype=torch.float32, device="cuda:0")
     render_device(scene, cfg, xyz=target, context=ctx)
     truth = params_vector(scene.atmosphere)
@@ -275,6 +277,7 @@ def main():
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + ["c4", "c5", "c5fit"])
     ap.add_argument("--mode", default="carried_is", choices=sorted(MODES))
     ap.add_argument("--layers", type=int, default=0, help="majorant shells (0: tuned)")
+    ap.add_argument("--shell-growth", type=float, default=0.0, help="shell thickness growth (0: tuned)")
     ap.add_argument("--autotune", action="store_true", help="re-tune launch parameters first (untimed)")
     ap.add_argument("--spp", type=int, default=0, help="override samples per pixel")
     ap.add_argument("--no-cpu", action="store_true")
@@ -304,11 +307,16 @@ def main():
                                                   phase_importance_sampling=True))
             autotune.apply(ctx, pysky.RenderConfig(), p)
             args.layers = args.layers or p["majorant_layers"]
+            args.shell_growth = p.get("shell_growth", 1.0) if args.shell_growth <= 0 else args.shell_growth
             out = bench_batch(args, ctx)
-        elif args.workload == "c5":
-            out = bench_adjoint(args, ctx)
         else:
-            out = run_fit(args, ctx)
+            # the adjoint and the fit use the large-render shell tuning
+            p = autotune.tuned(pysky.SceneData(wavelengths_nm=list(pysky.midpoint_grid(64))),
+                               pysky.RenderConfig(width=1024, height=512, carried_tau=True,
+                                                  phase_importance_sampling=True))
+            args.layers = args.layers or p["majorant_layers"]
+            args.shell_growth = p.get("shell_growth", 1.0) if args.shell_growth <= 0 else args.shell_growth
+            out = bench_adjoint(args, ctx) if args.workload == "c5" else run_fit(args, ctx)
         print(json.dumps(out))
         return
 
@@ -323,6 +331,8 @@ def main():
     params = autotune.tuned(scene, cfg)
     if args.layers:
         params["majorant_layers"] = args.layers
+    if args.shell_growth > 0:
+        params["shell_growth"] = args.shell_growth
     cfg = autotune.apply(ctx, cfg, params)
     args.layers = cfg.majorant_layers
     dev = torch.device("cuda:0")
