@@ -386,7 +386,9 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
     const bool phase_is = (s.flags & kFlagPhaseIS) != 0;
     int nx = NX_FINISH;
 
-    // 1. the event
+    // 1. the event; a ground hit or an accepted collision moves the vertex
+    const bool at_ground = c.ev == EV_BOUNDARY && c.kind == 0;
+    if (at_ground || c.ev == EV_HIT) advance_vertex(c, at_ground ? c.tmax : c.t);
     if (c.ev == EV_BOUNDARY) {
         if (c.kind == 1) {
             // exit: the sun disk is seen only by the camera ray (wgsl:745-749)
@@ -398,7 +400,6 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
                 }
             }
         } else if (c.kind == 0) {
-            advance_vertex(c, c.tmax);
             c.vtype = 1;
             c.nee_i = 0;
             c.nee_sum = 0.0f;
@@ -409,7 +410,6 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
         // kind 2: tracking watchdog or a main ray without boundary, the path ends
     } else if (c.ev == EV_HIT) {
         // tp and rw were updated when the collision was accepted
-        advance_vertex(c, c.t);
         if (c.tp > 0.0f) {
             c.vtype = 0;
             c.vin = c.d;
