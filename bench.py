@@ -115,21 +115,49 @@ def copy_calibration(mib):
     print(json.dumps({"copy_kib_per_dispatch": mib * 1024, "dispatches": 3}))
 
 
+SKYVIEW_ELEVATIONS = (-2.0, -1.0, 0.0, 1.0, 2.0, 5.0, 10.0, 20.0, 45.0, 90.0)
+
+
+def skyview_sweep(spp, mode):
+    """The reference's sky-view LUT bake (scripts/bake_skyview_elevations.ps1:2-35):
+    10 sun elevations, 256x256 sky-view LUT projection, 41 bands 380..780 nm
+    (10 nm wide), observer 0.2 km, one sun sample, uncapped depth."""
+    import pysky
+
+    lam = [380.0 + 10.0 * i for i in range(41)]
+    out = []
+    for el in SKYVIEW_ELEVATIONS:
+        scene = pysky.SceneData(sun_elevation_deg=el, wavelengths_nm=lam, weights_nm=[10.0] * 41)
+        cfg = pysky.RenderConfig(width=256, height=256, spp=spp, seed=0x5EC720260430, max_depth=0,
+                                 output_projection=pysky.OutputProjection.SkyViewLut, **MODES[mode])
+        out.append((scene, cfg))
+    return out
+
+
 def bench_batch(args, ctx):
     """Ragged batch (256 renders: sun U[-6, 90] deg, 64x32..512x256, 3..64
-    wavelengths in 400-700 nm, 256 spp) in one persistent launch replayed
-    from a HIP graph, against the same renders issued one by one."""
+    wavelengths in 400-700 nm, 256 spp) -- or the reference's sky-view LUT
+    elevation sweep -- in one persistent launch replayed from a HIP graph,
+    against the same renders issued one by one."""
     import numpy as np
     import torch
 
     from pysky.batch import ragged_batch, render_batch
     from pysky.torch_ops import render_device
 
-    batch = ragged_batch(n=256, seed=1, spp=args.spp or 256, max_depth=32)
+    if args.workload == "skyview":
+        batch = skyview_sweep(args.spp or 4096, args.mode)
+        what = (f"skyview: the reference's LUT bake, 10 sun elevations -2..90 deg, 256x256 sky-view LUT, "
+                f"41 bands 380-780 nm, {args.spp or 4096} spp, uncapped depth, one launch, HIP graph")
+    else:
+        batch = ragged_batch(n=256, seed=1, spp=args.spp or 256, max_depth=32)
+        what = ("c4: 256 ragged renders (sun U[-6,90] deg, 64x32..512x256, 3..64 lambda in "
+                "400-700 nm, 256 spp, depth<=32), one launch, HIP graph")
     layers, growth = args.layers, args.shell_growth
     batch = [(sc, dataclasses.replace(cfg, majorant_layers=layers, shell_growth=growth)) for sc, cfg in batch]
     samples = sum(cfg.width * cfg.height * cfg.spp * sc.band_count for sc, cfg in batch)
-    print(f"[bench] ragged batch: 256 renders, {samples:.3e} samples, layers={layers}", file=sys.stderr, flush=True)
+    print(f"[bench] batch of {len(batch)} renders, {samples:.3e} samples, layers={layers} growth={growth}",
+          file=sys.stderr, flush=True)
     for i in range(args.warmup):
         r = render_batch(batch, context=ctx)
         print(f"[bench] warmup {i}: kernel {r.stats.kernel_ms:.1f} ms", file=sys.stderr, flush=True)
@@ -160,15 +188,14 @@ def bench_batch(args, ctx):
     value = samples / wall
     single_rate = samples / single_wall
     return {
-        "metric": "ragged-batch path x lambda samples/s on 1 MI355X",
+        "metric": "batch path x lambda samples/s on 1 MI355X",
         "value": value,
         "unit": "path x lambda samples/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 ray set-up)",
-        "data": "synthetic (seeded ragged batch, analytic atmosphere)",
-        "config": {"workload": "c4: 256 ragged renders (sun U[-6,90] deg, 64x32..512x256, 3..64 lambda in "
-                               "400-700 nm, 256 spp, depth<=32), one launch, HIP graph",
-                   "estimator": "carried_is", "majorant_layers": layers, "shell_growth": growth},
+        "data": "synthetic (analytic atmosphere)",
+        "config": {"workload": what, "estimator": args.mode if args.workload == "skyview" else "carried_is",
+                   "majorant_layers": layers, "shell_growth": growth},
         "kernel_ms": float(np.mean(kms)),
         "latency_p50_ms": float(np.mean(p50)), "latency_p99_ms": float(np.mean(p99)),
         "single_render_rate": single_rate, "single_render_kernel_ms_sum": single_kernel_ms,
@@ -274,7 +301,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + ["c4", "c5", "c5fit"])
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + ["c4", "c5", "c5fit", "skyview"])
     ap.add_argument("--mode", default="carried_is", choices=sorted(MODES))
     ap.add_argument("--layers", type=int, default=0, help="majorant shells (0: tuned)")
     ap.add_argument("--shell-growth", type=float, default=0.0, help="shell thickness growth (0: tuned)")
@@ -299,9 +326,9 @@ def main():
 
     from pysky import autotune
 
-    if args.workload in ("c4", "c5", "c5fit"):
+    if args.workload in ("c4", "c5", "c5fit", "skyview"):
         ctx = pysky.Context(0)
-        if args.workload == "c4":
+        if args.workload in ("c4", "skyview"):
             p = autotune.tuned(pysky.SceneData(wavelengths_nm=list(pysky.midpoint_grid(64))),
                                pysky.RenderConfig(width=1024, height=512, carried_tau=True,
                                                   phase_importance_sampling=True))
