@@ -1,14 +1,17 @@
 // sky_render -- command-line driver mirroring sky-reference-render
-// (crates/sky-reference-render/src/main.rs:22-138): the same flags for the
-// render itself, the load / render / output / total timing lines, and the
-// film written as raw float32 files plus a small JSON manifest (the EXR/PNG
-// writers of film_output.rs are not part of this build).
+// (crates/sky-reference-render/src/main.rs:22-138): the same flags, the same
+// outputs (per-band EXRs, linear sRGB EXR, tonemapped PNG, asset.json) and
+// the load / render / output / total timing lines.  --bands, --max-depth and
+// --fast are extensions (band count of the analytic atmosphere, path depth
+// limit, carried optical depth + phase sampling).
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
 
+#include "film_output.hpp"
 #include "sky.hpp"
 
 namespace {
@@ -16,19 +19,30 @@ namespace {
 std::string format_duration(double s)  // main.rs:230-241
 {
     char buf[64];
-    if (s >= 1.0)
-        std::snprintf(buf, sizeof buf, "%.3fs", s);
-    else
+    if (s < 1.0) {
         std::snprintf(buf, sizeof buf, "%.1fms", s * 1e3);
+    } else if (s < 60.0) {
+        std::snprintf(buf, sizeof buf, "%.2fs", s);
+    } else {
+        double minutes = std::floor(s / 60.0);
+        std::snprintf(buf, sizeof buf, "%.0fm%.1fs", minutes, s - minutes * 60.0);
+    }
     return buf;
+}
+
+double since(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 void usage()
 {
     std::puts(
-        "sky_render [--width N] [--height N] [--spp N] [--seed N] [--sun-elevation-deg F]\n"
-        "           [--sun-azimuth-deg F] [--observer-altitude-km F] [--direct-light-samples N]\n"
-        "           [--sky-view-lut] [--bands N] [--max-depth N] [--fast] [--out DIR]");
+        "sky_render [--width N] [--height N] [--spp N] [--seed N] [--out DIR] [--data-dir DIR]\n"
+        "           [--sun-elevation-deg F] [--sun-azimuth-deg F] [--observer-altitude-km F]\n"
+        "           [--direct-light-samples N] [--png-exposure F] [--sky-view-lut] [--rebuild-rgb-only]\n"
+        "           [--bands N] [--max-depth N] [--fast]\n"
+        "--data-dir is accepted for flag compatibility: the analytic atmosphere reads no data files.");
 }
 
 }  // namespace
@@ -37,9 +51,11 @@ int main(int argc, char **argv)
 {
     auto total0 = std::chrono::steady_clock::now();
     sky::RenderConfig cfg;
-    bool lut = false;
+    cfg.spp = 1024;  // main.rs:30
+    bool lut = false, rebuild = false;
     bool width_set = false, height_set = false;
     uint32_t bands = 16;
+    float png_exposure = 0.05f;
     std::string out = "out";
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
@@ -54,15 +70,18 @@ int main(int argc, char **argv)
         else if (a == "--height") cfg.height = (uint32_t)std::strtoul(next(), nullptr, 10), height_set = true;
         else if (a == "--spp") cfg.spp = (uint32_t)std::strtoul(next(), nullptr, 10);
         else if (a == "--seed") cfg.seed = std::strtoull(next(), nullptr, 0);
+        else if (a == "--out") out = next();
+        else if (a == "--data-dir") next();
         else if (a == "--sun-elevation-deg") cfg.sun_elevation_deg = std::strtof(next(), nullptr);
         else if (a == "--sun-azimuth-deg") cfg.sun_azimuth_deg = std::strtof(next(), nullptr);
         else if (a == "--observer-altitude-km") cfg.observer_altitude_km = std::strtof(next(), nullptr);
         else if (a == "--direct-light-samples") cfg.direct_light_samples = (uint32_t)std::strtoul(next(), nullptr, 10);
+        else if (a == "--png-exposure") png_exposure = std::strtof(next(), nullptr);
         else if (a == "--sky-view-lut") lut = true;
+        else if (a == "--rebuild-rgb-only") rebuild = true;
         else if (a == "--bands") bands = (uint32_t)std::strtoul(next(), nullptr, 10);
         else if (a == "--max-depth") cfg.max_depth = (uint32_t)std::strtoul(next(), nullptr, 10);
         else if (a == "--fast") cfg.flags |= SKY_FLAG_CARRIED_TAU | SKY_FLAG_PHASE_IS;
-        else if (a == "--out") out = next();
         else {
             usage();
             return 2;
@@ -73,41 +92,40 @@ int main(int argc, char **argv)
     if (!height_set) cfg.height = lut ? 256 : 1024;
     cfg.output_projection = lut ? sky::OutputProjection::SkyViewLut : sky::OutputProjection::Panorama;
 
-    auto load0 = std::chrono::steady_clock::now();
-    sky::SceneData scene = sky::load_scene_data(cfg.sun_elevation_deg, cfg.sun_azimuth_deg, bands);
-    double load_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - load0).count();
-    std::printf("config: gpu-integrator output=%s direct-light-samples=%u\n", sky::label(cfg.output_projection),
-                cfg.direct_light_samples);
-    auto render0 = std::chrono::steady_clock::now();
     try {
+        auto load0 = std::chrono::steady_clock::now();
+        sky::SceneData scene = sky::load_scene_data(cfg.sun_elevation_deg, cfg.sun_azimuth_deg, bands);
+        double load_s = since(load0);
+
+        if (rebuild) {  // main.rs:85-104
+            auto out0 = std::chrono::steady_clock::now();
+            sky::Film film = sky::read_film_from_band_exrs(out, scene);
+            sky::write_rgb_outputs(film, scene, out, png_exposure);
+            sky::update_asset_colorimetry(out, scene);
+            double out_s = since(out0);
+            std::printf("rebuilt rgb outputs from %zu band EXRs in %s\n", scene.wavelengths_nm.size(), out.c_str());
+            std::printf("timing: load=%s output=%s total=%s\n", format_duration(load_s).c_str(),
+                        format_duration(out_s).c_str(), format_duration(since(total0)).c_str());
+            return 0;
+        }
+
+        std::printf("config: gpu-integrator output=%s direct-light-samples=%u\n", sky::label(cfg.output_projection),
+                    cfg.direct_light_samples);
+        std::fflush(stdout);
+        auto render0 = std::chrono::steady_clock::now();
         sky::Film film = sky::render(scene, cfg);
-        double render_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - render0).count();
+        double render_s = since(render0);
+
         auto out0 = std::chrono::steady_clock::now();
-        std::string mk = "mkdir -p '" + out + "'";
-        if (std::system(mk.c_str()) != 0) throw std::runtime_error("cannot create " + out);
-        FILE *f = std::fopen((out + "/film_f32.raw").c_str(), "wb");
-        FILE *g = std::fopen((out + "/xyz_f32.raw").c_str(), "wb");
-        FILE *m = std::fopen((out + "/asset.json").c_str(), "w");
-        if (!f || !g || !m) throw std::runtime_error("cannot write outputs in " + out);
-        std::fwrite(film.values().data(), sizeof(float), film.values().size(), f);
-        std::fwrite(film.xyz().data(), sizeof(float), film.xyz().size(), g);
-        std::fprintf(m, "{\"width\": %u, \"height\": %u, \"bands\": %u, \"spp\": %u, \"projection\": \"%s\", "
-                        "\"film\": \"film_f32.raw\", \"xyz\": \"xyz_f32.raw\", \"wavelengths_nm\": [",
-                     film.width(), film.height(), film.band_count(), cfg.spp, sky::label(cfg.output_projection));
-        for (size_t i = 0; i < scene.wavelengths_nm.size(); ++i)
-            std::fprintf(m, "%s%.6f", i ? ", " : "", scene.wavelengths_nm[i]);
-        std::fprintf(m, "]}\n");
-        std::fclose(f);
-        std::fclose(g);
-        std::fclose(m);
-        double out_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - out0).count();
-        double total_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - total0).count();
+        sky::write_outputs(film, scene, out, png_exposure);
+        sky::write_asset_manifest(cfg, scene, out);
+        double out_s = since(out0);
         std::printf("wrote %ux%u %s with %u spp to %s\n", film.width(), film.height(),
                     sky::label(cfg.output_projection), cfg.spp, out.c_str());
         std::printf("timing: load=%s render=%s output=%s total=%s\n", format_duration(load_s).c_str(),
                     format_duration(render_s).c_str(), format_duration(out_s).c_str(),
-                    format_duration(total_s).c_str());
-    } catch (const sky::RenderError &e) {
+                    format_duration(since(total0)).c_str());
+    } catch (const std::exception &e) {
         std::fprintf(stderr, "error: %s\n", e.what());
         return 1;
     }

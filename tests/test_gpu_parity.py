@@ -172,19 +172,34 @@ def test_optical_depth_matches_oracle(gpu_context):
 
 
 def test_cli_renders(tmp_path):
-    """sky_render CLI (main.rs:58-138 counterpart): writes the film, the XYZ
-    image and the manifest, and prints the timing line."""
+    """sky_render CLI (main.rs:58-138 counterpart): per-band EXRs, the RGB EXR,
+    the PNG and the manifest, the timing line; --rebuild-rgb-only then
+    rebuilds the same RGB EXR from the band files."""
     import json
     import os
     import subprocess
 
+    from exr_png import read_exr, read_png_rgb8
+
     exe = os.path.join(os.path.dirname(pysky.LIB_PATH), "..", "bin", "sky_render")
-    r = subprocess.run([exe, "--width", "32", "--height", "16", "--spp", "4", "--sun-elevation-deg", "10",
-                        "--bands", "16", "--max-depth", "8", "--out", str(tmp_path)],
-                       capture_output=True, text=True, timeout=120)
+    args = ["--width", "32", "--height", "16", "--spp", "4", "--sun-elevation-deg", "10", "--bands", "16",
+            "--max-depth", "8", "--out", str(tmp_path)]
+    r = subprocess.run([exe, *args], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
+    assert "wrote 32x16 panorama with 4 spp" in r.stdout
     assert "timing: load=" in r.stdout and "render=" in r.stdout
     m = json.load(open(tmp_path / "asset.json"))
-    film = np.fromfile(tmp_path / "film_f32.raw", dtype=np.float32)
-    assert film.size == 32 * 16 * 16 and m["bands"] == 16
-    assert np.all(np.isfinite(film)) and np.any(film > 0)
+    assert m["kind"] == "spectral_panorama_v0" and m["dimensions"] == [32, 16] and m["spp"] == 4
+    assert len(m["files"]["band_exrs"]) == 16 and m["band_centers_nm"][0] == 392.5
+    bands = np.stack([read_exr(tmp_path / f)["R"] for f in m["files"]["band_exrs"]], axis=-1)
+    assert bands.shape == (16, 32, 16)
+    assert np.all(np.isfinite(bands)) and np.all(bands >= 0) and np.any(bands > 0)
+    rgb = read_exr(tmp_path / "sky_rgb.exr")
+    assert read_png_rgb8(tmp_path / "sky_rgb.png").shape == (16, 32, 3)
+    os.remove(tmp_path / "sky_rgb.exr")
+    r = subprocess.run([exe, "--rebuild-rgb-only", "--bands", "16", "--out", str(tmp_path)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    again = read_exr(tmp_path / "sky_rgb.exr")
+    for c in "RGB":
+        assert np.array_equal(rgb[c], again[c])
