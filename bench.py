@@ -296,6 +296,22 @@ def run_fit(args, ctx):
     }
 
 
+def accuracy_vs_fp64(xyz, w, args):
+    """Image relative RMSE (over the 64x32 grid x = 16i+8, y = 16j+8) of the
+    rendered XYZ against the FP64 oracle's 65536-spp render of the same scene.
+    The figure includes the FP64 render's own Monte Carlo noise (`ref_noise`)."""
+    import numpy as np
+
+    if args.workload != "c2":
+        raise SystemExit("--accuracy compares the 1024x512 sun-5-degree render only")
+    ref = np.load(os.path.join(ROOT, "tests", "golden", "c2_fp64_reference.npz"))
+    got = xyz.reshape(-1, 3).cpu().numpy().astype(np.float64)[ref["pixels"]]
+    rms = float(np.sqrt(np.mean(ref["xyz"] ** 2)))
+    return {"rel_rmse": round(float(np.sqrt(np.mean((got - ref["xyz"]) ** 2))) / rms, 6),
+            "ref_noise": round(float(np.sqrt(np.mean(ref["xyz_se"] ** 2))) / rms, 6),
+            "ref_spp": int(ref["spp"]), "spp": w["spp"], "pixels": int(ref["pixels"].size)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -308,6 +324,9 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="re-tune launch parameters first (untimed)")
     ap.add_argument("--spp", type=int, default=0, help="override samples per pixel")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--accuracy", action="store_true",
+                    help="c2 only: relative RMSE of the last render's XYZ against the 65536-spp FP64 render "
+                         "of tests/golden/c2_fp64_reference.npz")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device copies of MIB MiB (scale for the FETCH/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -351,7 +370,8 @@ def main():
     if args.spp:
         w["spp"] = args.spp
     scene = make_scene(w)
-    cfg = make_config(w, args.mode, 64)
+    # the FP64 reference render used seed 1: an accuracy run draws other random numbers
+    cfg = make_config(w, args.mode, 64, seed=2 if args.accuracy else 1)
     ctx = pysky.Context(0)
     if args.autotune:
         autotune.autotune(scene, cfg, ctx, log=lambda m: print(m, file=sys.stderr, flush=True))
@@ -426,6 +446,8 @@ def main():
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
+    if args.accuracy:
+        out["accuracy"] = accuracy_vs_fp64(xyz, w, args)
     print(json.dumps(out))
 
 

@@ -145,6 +145,18 @@ def lib():
         L.orc_sun_cone_dir.restype = C.c_double
         L.orc_cosine_dir.argtypes = [dp, C.c_double, C.c_double, dp]
         L.orc_cosine_dir.restype = C.c_double
+        fp = C.POINTER(C.c_float)
+        u8 = C.c_void_p
+        L.orc_band_widths.argtypes = [fp, C.c_uint32, dp]
+        L.orc_quadrature_weights.argtypes = [fp, C.c_uint32, dp, u8, C.c_uint32, C.c_uint32, dp]
+        L.orc_quadrature_weights.restype = C.c_int
+        L.orc_xyz_to_oklab.argtypes = [dp, dp]
+        L.orc_percentile.argtypes = [dp, C.c_uint64, C.c_double]
+        L.orc_percentile.restype = C.c_double
+        L.orc_reference_oklab.argtypes = [fp, C.c_uint32, C.c_uint64, dp, dp]
+        L.orc_reference_oklab.restype = C.c_double
+        L.orc_evaluate_combos.argtypes = [fp, C.c_uint32, C.c_uint64, dp, C.c_double, fp, dp, u8, C.c_uint32,
+                                          C.c_uint64, dp, C.c_void_p, C.c_uint64, C.c_int, dp]
         _lib = L
     return _lib
 
@@ -421,3 +433,69 @@ def cosine_dir(n, u: float, xi: float):
     out = np.zeros(3)
     pdf = float(lib().orc_cosine_dir(_dp(a), u, xi, _dp(out)))
     return out, pdf
+
+
+# ------------------------------------------------ sparse-wavelength combo scan
+# (crates/sky-wavelength-optimizer/src/main.rs, restated in scan_oracle.c)
+
+SCAN_VORONOI, SCAN_CMF_MASS, SCAN_CMF_MOMENTS = 0, 1, 2
+SCAN_FIELDS = ("mean", "rmse", "p50", "p90", "p95", "p99", "max", "score")
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def band_widths(wl) -> np.ndarray:
+    w = np.ascontiguousarray(wl, dtype=np.float32)
+    out = np.zeros(w.size)
+    lib().orc_band_widths(_fp(w), w.size, _dp(out))
+    return out
+
+
+def quadrature_weights(wl, combo, strategy, cie=None) -> np.ndarray:
+    """main.rs:735-1070 for one combo -> [count] weights (nm)."""
+    w = np.ascontiguousarray(wl, dtype=np.float32)
+    c = np.ascontiguousarray(combo, dtype=np.uint8)
+    ci = np.ascontiguousarray(np.zeros((w.size, 3)) if cie is None else cie, dtype=np.float64)
+    out = np.zeros(8)
+    if lib().orc_quadrature_weights(_fp(w), w.size, _dp(ci), c.ctypes.data, c.size, strategy, _dp(out)):
+        raise ValueError("invalid combo")
+    return out[:c.size]
+
+
+def xyz_to_oklab(xyz) -> np.ndarray:
+    x = np.ascontiguousarray(xyz, dtype=np.float64)
+    out = np.zeros(3)
+    lib().orc_xyz_to_oklab(_dp(x), _dp(out))
+    return out
+
+
+def percentile(values, q: float) -> float:
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    return float(lib().orc_percentile(_dp(v), v.size, q))
+
+
+def reference_oklab(values, cie):
+    """main.rs:276-341 -> (reference Oklab [n, 3], p99 Y scale)."""
+    v = np.ascontiguousarray(values, dtype=np.float32)
+    ci = np.ascontiguousarray(cie, dtype=np.float64)
+    out = np.zeros((v.shape[0], 3))
+    scale = lib().orc_reference_oklab(_fp(v), v.shape[1], v.shape[0], _dp(ci), _dp(out))
+    return out, float(scale)
+
+
+def evaluate_combos(values, ref_oklab, scale, wl, cie, combos, weights, idx=None, exact=False) -> np.ndarray:
+    """main.rs:414-488 per combo -> [n_combos, 8] (SCAN_FIELDS)."""
+    v = np.ascontiguousarray(values, dtype=np.float32)
+    r = np.ascontiguousarray(ref_oklab, dtype=np.float64)
+    w = np.ascontiguousarray(wl, dtype=np.float32)
+    ci = np.ascontiguousarray(cie, dtype=np.float64)
+    cb = np.ascontiguousarray(np.atleast_2d(combos), dtype=np.uint8)
+    wt = np.ascontiguousarray(np.atleast_2d(weights), dtype=np.float64)
+    ix = None if idx is None else np.ascontiguousarray(idx, dtype=np.uint32)
+    out = np.zeros((cb.shape[0], 8))
+    lib().orc_evaluate_combos(_fp(v), v.shape[1], v.shape[0], _dp(r), scale, _fp(w), _dp(ci), cb.ctypes.data,
+                              cb.shape[1], cb.shape[0], _dp(wt), None if ix is None else ix.ctypes.data,
+                              0 if ix is None else ix.size, int(exact), _dp(out))
+    return out

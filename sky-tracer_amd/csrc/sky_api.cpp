@@ -160,6 +160,8 @@ sky_status run_render(sky_context *ctx, sky::HostScene &hs, hipStream_t stream,
 
 }  // namespace
 
+sky_status sky::set_error(sky_status st, const std::string &msg) { return fail(st, msg); }
+
 extern "C" {
 
 const char *sky_last_error(void) { return g_last_error.c_str(); }

@@ -95,4 +95,7 @@ hipError_t launch_adjoint(const DevJob *d_jobs, const WavePlan &p, uint32_t *met
 hipError_t launch_optical_depth(const DevScene &s, const double *rays, uint32_t n,
                                 const double *dev_lambda_d, double *tau, hipStream_t stream);
 
+// Records the message returned by sky_last_error() and passes st through.
+sky_status set_error(sky_status st, const std::string &msg);
+
 }  // namespace sky

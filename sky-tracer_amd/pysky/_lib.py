@@ -137,6 +137,15 @@ class BatchStats(C.Structure):
 
 BATCH_NO_GRAPH = 1
 
+class ScanStats(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("total_ms", C.c_double), ("evaluations", C.c_uint64)]
+
+
+class ComboStats(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("mean", "rmse", "p50", "p90", "p95", "p99", "max")] + [
+        ("sample_count", C.c_uint64)]
+
+
 _lib = None
 
 
@@ -175,6 +184,16 @@ def lib() -> C.CDLL:
                                       C.c_int),
         "sky_render_batch": ([ctxp, C.POINTER(BatchItem), C.c_uint32, C.c_uint32, dp, C.POINTER(BatchStats)],
                              C.c_int),
+        "sky_scan_dataset_create": ([C.c_int, fp, C.c_uint32, C.c_uint64, fp, C.POINTER(ctxp)], C.c_int),
+        "sky_scan_dataset_destroy": ([ctxp], None),
+        "sky_scan_combos": ([ctxp, C.c_void_p, fp, C.c_uint32, C.c_uint64, C.c_void_p, C.c_uint64, dp, dp,
+                             C.POINTER(ScanStats)], C.c_int),
+        "sky_refine_combos": ([ctxp, C.c_void_p, fp, C.c_uint32, C.c_uint64, C.c_void_p, C.c_uint64,
+                               C.POINTER(ComboStats), C.POINTER(ScanStats)], C.c_int),
+        "sky_band_widths": ([fp, C.c_uint32, dp], C.c_int),
+        "sky_quadrature_weights": ([fp, C.c_uint32, dp, C.c_void_p, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32,
+                                    dp], C.c_int),
+        "sky_cie_1931_2deg": ([fp, C.c_uint32, dp], C.c_int),
         "sky_last_error": ([], C.c_char_p),
         "sky_version": ([], C.c_char_p),
     }

@@ -27,8 +27,9 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c2_
 
 def test_full_render_matches_fp64_oracle(gpu_context):
     ref = np.load(GOLDEN)
+    # seed 2: the FP64 render used seed 1, whose first samples would share random numbers
     scene, cfg, _ = case(width=1024, height=512, n_lambda=64, spp=1024, sun=5.0, max_depth=32,
-                         mode="carried_is", layers=8)
+                         mode="carried_is", layers=8, seed=2)
     assert np.allclose(scene.wavelengths_nm, ref["wavelengths_nm"])
     print(f"[gpu] render 1024x512 nl=64 spp=1024 carried_is L8", flush=True)
     f = pysky.render(scene, cfg, want_film=True, want_m2=True, want_xyz=True, context=gpu_context)
