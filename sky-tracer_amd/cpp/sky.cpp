@@ -44,6 +44,14 @@ SceneData load_scene_data(float sun_elevation_deg, float sun_azimuth_deg, uint32
     sky_default_atmosphere(&s.atmosphere);
     s.sun_elevation_deg = sun_elevation_deg;
     s.sun_azimuth_deg = sun_azimuth_deg;
+    if (band_count == 0) {
+        // the reference's fixed grid: 41 bands at 380, 390, ..., 780 nm, 10 nm wide (spectrum.rs:5, 166-175)
+        for (uint32_t i = 0; i < 41; ++i) {
+            s.wavelengths_nm.push_back(380.0 + 10.0 * i);
+            s.weights_nm.push_back(10.0);
+        }
+        return s;
+    }
     double step = 400.0 / band_count;
     for (uint32_t i = 0; i < band_count; ++i) s.wavelengths_nm.push_back(380.0 + step * (i + 0.5));
     return s;

@@ -74,7 +74,9 @@ private:
     Kind kind_;
 };
 
-// load_scene_data for the analytic atmosphere: midpoint bands over 380-780 nm.
+// load_scene_data for the analytic atmosphere: band_count midpoint bands over
+// 380-780 nm, or with band_count 0 the reference's 41 bands at 380..780 nm
+// every 10 nm.
 SceneData load_scene_data(float sun_elevation_deg, float sun_azimuth_deg, uint32_t band_count = 16);
 
 // integrator::render: validates, renders on device 0 (one long-lived

@@ -1,8 +1,9 @@
 // sky_render -- command-line driver mirroring sky-reference-render
 // (crates/sky-reference-render/src/main.rs:22-138): the same flags, the same
 // outputs (per-band EXRs, linear sRGB EXR, tonemapped PNG, asset.json) and
-// the load / render / output / total timing lines.  --bands, --max-depth and
-// --fast are extensions (band count of the analytic atmosphere, path depth
+// the load / render / output / total timing lines.  The default band set is
+// the reference's 41 bands (380..780 nm every 10 nm).  --bands N (N midpoint
+// bands over 380-780 nm), --max-depth and --fast are extensions (path depth
 // limit, carried optical depth + phase sampling).
 #include <chrono>
 #include <cmath>
@@ -54,7 +55,7 @@ int main(int argc, char **argv)
     cfg.spp = 1024;  // main.rs:30
     bool lut = false, rebuild = false;
     bool width_set = false, height_set = false;
-    uint32_t bands = 16;
+    uint32_t bands = 0;  // 0: the reference's 41-band grid
     float png_exposure = 0.05f;
     std::string out = "out";
     for (int i = 1; i < argc; ++i) {

@@ -39,6 +39,10 @@ def _run(*args):
     return r
 
 
+def _rebuild(out, *args):
+    return _run("--rebuild-rgb-only", "--bands", "16", "--out", str(out), *args)
+
+
 def _write_bands(out, film, half=False):
     os.makedirs(out / "bands", exist_ok=True)
     for b, nm in enumerate(_bands(film.shape[2])):
@@ -93,7 +97,7 @@ def test_rebuild_rgb_from_band_exrs(tmp_path):
                                              "xyz_from_xyz": [[1, 0, 0], [0, 1, 0], [0, 0, 1]]}},
            "files": {"rgb_exr": "sky_rgb.exr", "rgb_png": "sky_rgb.png", "band_exrs": []}}
     (tmp_path / "asset.json").write_text(json.dumps(old, indent=2))
-    r = _run("--rebuild-rgb-only", "--out", str(tmp_path))
+    r = _rebuild(tmp_path)
     assert r.returncode == 0, r.stderr
     assert "rebuilt rgb outputs from 16 band EXRs" in r.stdout and "timing: load=" in r.stdout
 
@@ -125,7 +129,7 @@ def test_rebuild_reads_half_float_bands(tmp_path):
     film, lam = _film(h=4, w=6)
     film = film.astype(np.float16).astype(np.float32)
     _write_bands(tmp_path, film, half=True)
-    r = _run("--rebuild-rgb-only", "--out", str(tmp_path))
+    r = _rebuild(tmp_path)
     assert r.returncode == 0, r.stderr
     assert not (tmp_path / "asset.json").exists()  # no manifest: nothing is created
     exr = read_exr(tmp_path / "sky_rgb.exr")
@@ -136,7 +140,7 @@ def test_rebuild_reads_half_float_bands(tmp_path):
 def test_rebuild_png_exposure(tmp_path):
     film, lam = _film(h=3, w=5)
     _write_bands(tmp_path, film)
-    r = _run("--rebuild-rgb-only", "--png-exposure", "0.4", "--out", str(tmp_path))
+    r = _rebuild(tmp_path, "--png-exposure", "0.4")
     assert r.returncode == 0, r.stderr
     _, png_want, _, _ = _expected(film, lam, exposure=0.4)
     assert np.abs(read_png_rgb8(tmp_path / "sky_rgb.png").astype(np.int32) - png_want).max() <= 1
@@ -147,13 +151,16 @@ def test_rebuild_rejects_mismatched_band_sizes(tmp_path):
     _write_bands(tmp_path, film)
     v = np.zeros((5, 6), np.float32)
     write_exr(tmp_path / "bands" / _band_name(_bands()[3]), {"R": v, "G": v, "B": v})
-    r = _run("--rebuild-rgb-only", "--out", str(tmp_path))
+    r = _rebuild(tmp_path)
     assert r.returncode == 1 and "other dimensions" in r.stderr
 
 
 def test_rebuild_missing_band(tmp_path):
-    r = _run("--rebuild-rgb-only", "--out", str(tmp_path))
+    r = _rebuild(tmp_path)
     assert r.returncode == 1 and "sky_392nm.exr" in r.stderr
+    # without --bands the reference's 41-band grid (380..780 nm every 10 nm) is expected
+    r = _run("--rebuild-rgb-only", "--out", str(tmp_path))
+    assert r.returncode == 1 and "sky_380nm.exr" in r.stderr
 
 
 def test_unknown_flag_prints_usage():
