@@ -296,6 +296,130 @@ def run_fit(args, ctx):
     }
 
 
+def scan_flops_per_eval(primaries):
+    """Algorithmic FP64 operations of one (combo, sample) evaluation of the
+    reference's evaluate_combo (main.rs:443-458): per channel v * w / width and
+    three multiply-adds (8); XYZ / scale (3); XYZ -> LMS (15); three cube roots
+    (3); LMS' -> Lab (15); the distance (3 sub, 3 mul, 2 add, 1 sqrt = 9); the
+    two running sums (3).  Cube root and square root count as one each."""
+    return 8 * primaries + 3 + 15 + 3 + 15 + 9 + 3
+
+
+def scan_dataset(spp):
+    """The sky-view LUT bake of 10 elevations x 256^2 x 41 bands (655,360
+    samples) rendered at `spp` samples, with the reference colour of every
+    sample (main.rs:276-341)."""
+    import numpy as np
+
+    import pysky
+    from pysky import scan
+
+    ctx = pysky.Context(0)
+    vals = []
+    for scene, cfg in skyview_sweep(spp, "carried_is"):
+        vals.append(pysky.render(scene, cfg, want_xyz=False, context=ctx).values().reshape(-1, 41))
+    v = np.concatenate(vals).astype(np.float32)
+    wl = np.array([380.0 + 10.0 * i for i in range(41)], np.float32)
+    cie = scan.cie_1931_2deg(wl)
+    xyz = v.astype(np.float64) @ cie
+    y = np.sort(xyz[:, 1][np.isfinite(xyz[:, 1])])
+    scale = max(float(y[int(np.floor((y.size - 1) * 0.99 + 0.5))]), 1e-12)
+    return v, wl, cie, scale, scan.xyz_to_oklab(xyz / scale)
+
+
+def bench_scan(args):
+    """The wavelength optimizer's screening pass (main.rs:156-199): every
+    combo of `primaries` bands from the 41-band grid scored on 65,536 evenly
+    spread samples of the 10-elevation sky-view LUT bake, in one launch with
+    every operand resident in HBM."""
+    import numpy as np
+    import torch
+
+    from pysky import scan
+
+    P = args.primaries
+    v, wl, cie, scale, ref = scan_dataset(args.spp or 64)
+    ds = scan.ScanDataset(v, ref)
+    import itertools
+    combos = np.fromiter(itertools.chain.from_iterable(itertools.combinations(range(41), P)), np.uint8).reshape(-1, P)
+    t0 = time.perf_counter()
+    w = scan.quadrature_weights(wl, combos, scan.VORONOI)
+    coef = scan.combo_coefficients(combos, w, scan.band_widths(wl), cie, scale)
+    prep_s = time.perf_counter() - t0
+    idx = scan.screening_indices(v.shape[0], 65536)
+    dev = torch.device("cuda:0")
+    tc = torch.from_numpy(combos).to(dev)
+    tf = torch.from_numpy(coef).to(dev)
+    ti = torch.from_numpy(idx.astype(np.int32)).to(dev)
+    ts = torch.zeros(len(combos), dtype=torch.float64, device=dev)
+    tq = torch.zeros_like(ts)
+    evals = len(combos) * idx.size
+    print(f"[bench] scan: {len(combos)} {P}-band combos x {idx.size} samples = {evals:.3e} evaluations "
+          f"(weights+coefficients on the host {prep_s:.1f} s)", file=sys.stderr, flush=True)
+    for i in range(args.warmup):
+        st = ds.scan_device(tc, tf, ts, tq, samples=ti)
+        print(f"[bench] warmup {i}: kernel {st.kernel_ms:.1f} ms", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        st = ds.scan_device(tc, tf, ts, tq, samples=ti)
+        kernel_ms.append(st.kernel_ms)
+        print(f"[bench] step {i}: kernel {st.kernel_ms:.1f} ms", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    value = evals * args.steps / elapsed
+    avg_s = sum(kernel_ms) / len(kernel_ms) / 1e3
+    f_eval = scan_flops_per_eval(P)
+    achieved = f_eval * evals / avg_s / 1e12
+    rmse = np.sqrt(tq.cpu().numpy() / idx.size)
+    best = combos[int(np.argmin(rmse))]
+    cpu = None
+    if not args.no_cpu:
+        cpu = scan_cpu_baseline(v, wl, cie, scale, ref, combos, w, idx)
+    out = {
+        "metric": "combo x sample evaluations/s (wavelength-optimizer screening) on 1 MI355X",
+        "value": value, "unit": "evaluations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (f64 sums)", "data": "synthetic (sky-view LUT bake of this build)",
+        "config": {"workload": f"scan: all {len(combos)} {P}-band combos of the 41-band grid, Voronoi weights, "
+                               f"65536 of 655360 sky-view LUT samples ({args.spp or 64}-spp bake), one launch",
+                   "parallelism": "1 GPU"},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 5), "traffic": None,
+                     "flops_per_eval": f_eval, "kernel": f"k_scan<{P}>", "kernel_ms_avg": round(avg_s * 1e3, 2)},
+        "cpu_baseline": cpu,
+        "best_screened_combo_nm": [float(wl[b]) for b in best],
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
+    return out
+
+
+def scan_cpu_baseline(v, wl, cie, scale, ref, combos, w, idx, target_s=15.0):
+    """The FP64 oracle's evaluate_combo on the host cores over a bounded,
+    evenly spread subset of the combos, on the same screening samples."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+
+    import oracle as O
+
+    threads = max(1, min(os.cpu_count() or 1, 16))
+
+    def run(sel):
+        t0 = time.perf_counter()
+        O.evaluate_combos(v, ref, scale, wl, cie, combos[sel], w[sel], idx=idx, threads=threads)
+        return time.perf_counter() - t0
+
+    probe = np.linspace(0, len(combos) - 1, 64).astype(np.int64)
+    rate = 64 / max(run(probe), 1e-6)
+    sel = np.linspace(0, len(combos) - 1, max(64, min(len(combos), int(target_s * rate)))).astype(np.int64)
+    dt = run(sel)
+    return {"value": len(sel) * idx.size / dt, "unit": "evaluations/s", "cores": threads, "kind": "port",
+            "sample": f"oracle FP64 evaluate_combo, {len(sel)} of {len(combos)} combos (evenly spread) x "
+                      f"{idx.size} samples, {dt:.1f} s on {threads} threads"}
+
+
 def accuracy_vs_fp64(xyz, w, args):
     """Image relative RMSE (over the 64x32 grid x = 16i+8, y = 16j+8) of the
     rendered XYZ against the FP64 oracle's 65536-spp render of the same scene.
@@ -317,13 +441,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + ["c4", "c5", "c5fit", "skyview"])
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + ["c4", "c5", "c5fit", "skyview", "scan"])
     ap.add_argument("--mode", default="carried_is", choices=sorted(MODES))
     ap.add_argument("--layers", type=int, default=0, help="majorant shells (0: tuned)")
     ap.add_argument("--shell-growth", type=float, default=0.0, help="shell thickness growth (0: tuned)")
     ap.add_argument("--autotune", action="store_true", help="re-tune launch parameters first (untimed)")
     ap.add_argument("--spp", type=int, default=0, help="override samples per pixel")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--primaries", type=int, default=6, help="scan: bands per combo (6: all 4,496,388 combos)")
     ap.add_argument("--accuracy", action="store_true",
                     help="c2 only: relative RMSE of the last render's XYZ against the 65536-spp FP64 render "
                          "of tests/golden/c2_fp64_reference.npz")
@@ -345,6 +470,9 @@ def main():
 
     from pysky import autotune
 
+    if args.workload == "scan":
+        print(json.dumps(bench_scan(args)))
+        return
     if args.workload in ("c4", "c5", "c5fit", "skyview"):
         ctx = pysky.Context(0)
         if args.workload in ("c4", "skyview"):

@@ -68,6 +68,14 @@ sky_status sky_scan_combos(sky_scan_dataset *ds, const uint8_t *combos, const fl
                            uint64_t n_combos, const uint32_t *samples, uint64_t n_samples, double *sum,
                            double *sum_sq, sky_scan_stats *stats);
 
+/* Same with device pointers (e.g. torch tensors) for combos, coef, samples and
+ * the sums, ordered on `stream` (hipStream_t, NULL = the dataset's stream)
+ * and synchronous on return.  Out-of-range band or sample indices are caught
+ * on the device (clamped, then reported as SKY_ERR_INVALID_CONFIG). */
+sky_status sky_scan_combos_device(sky_scan_dataset *ds, const uint8_t *d_combos, const float *d_coef, uint32_t count,
+                                  uint64_t n_combos, const uint32_t *d_samples, uint64_t n_samples, double *d_sum,
+                                  double *d_sum_sq, void *stream, sky_scan_stats *stats);
+
 /* ComboResult statistics (main.rs:97-110) of one combo. */
 typedef struct sky_combo_stats {
     double mean, rmse, p50, p90, p95, p99, max;

@@ -156,7 +156,7 @@ def lib():
         L.orc_reference_oklab.argtypes = [fp, C.c_uint32, C.c_uint64, dp, dp]
         L.orc_reference_oklab.restype = C.c_double
         L.orc_evaluate_combos.argtypes = [fp, C.c_uint32, C.c_uint64, dp, C.c_double, fp, dp, u8, C.c_uint32,
-                                          C.c_uint64, dp, C.c_void_p, C.c_uint64, C.c_int, dp]
+                                          C.c_uint64, dp, C.c_void_p, C.c_uint64, C.c_int, C.c_int, dp]
         _lib = L
     return _lib
 
@@ -485,8 +485,10 @@ def reference_oklab(values, cie):
     return out, float(scale)
 
 
-def evaluate_combos(values, ref_oklab, scale, wl, cie, combos, weights, idx=None, exact=False) -> np.ndarray:
-    """main.rs:414-488 per combo -> [n_combos, 8] (SCAN_FIELDS)."""
+def evaluate_combos(values, ref_oklab, scale, wl, cie, combos, weights, idx=None, exact=False,
+                    threads=0) -> np.ndarray:
+    """main.rs:414-488 per combo -> [n_combos, 8] (SCAN_FIELDS), on `threads`
+    host threads (0: OpenMP's default)."""
     v = np.ascontiguousarray(values, dtype=np.float32)
     r = np.ascontiguousarray(ref_oklab, dtype=np.float64)
     w = np.ascontiguousarray(wl, dtype=np.float32)
@@ -497,5 +499,5 @@ def evaluate_combos(values, ref_oklab, scale, wl, cie, combos, weights, idx=None
     out = np.zeros((cb.shape[0], 8))
     lib().orc_evaluate_combos(_fp(v), v.shape[1], v.shape[0], _dp(r), scale, _fp(w), _dp(ci), cb.ctypes.data,
                               cb.shape[1], cb.shape[0], _dp(wt), None if ix is None else ix.ctypes.data,
-                              0 if ix is None else ix.size, int(exact), _dp(out))
+                              0 if ix is None else ix.size, int(exact), int(threads), _dp(out))
     return out

@@ -8,6 +8,7 @@
  * product links, loads or calls this code.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -307,9 +308,10 @@ void orc_evaluate_combo(const float *values, uint32_t n_bands, uint64_t n_sample
 void orc_evaluate_combos(const float *values, uint32_t n_bands, uint64_t n_samples, const double *ref_oklab,
                          double scale, const float *wl, const double *cie, const uint8_t *combos, uint32_t count,
                          uint64_t n_combos, const double *weights, const uint32_t *idx, uint64_t n_idx, int exact,
-                         double *out)
+                         int threads, double *out)
 {
-#pragma omp parallel for schedule(dynamic, 4)
+    if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 4) num_threads(threads)
     for (int64_t c = 0; c < (int64_t)n_combos; ++c)
         orc_evaluate_combo(values, n_bands, n_samples, ref_oklab, scale, wl, cie, combos + (uint64_t)c * count, count,
                            weights + (uint64_t)c * count, idx, n_idx, exact, out + 8 * (uint64_t)c);

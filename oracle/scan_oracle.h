@@ -36,7 +36,7 @@ void orc_evaluate_combo(const float *values, uint32_t n_bands, uint64_t n_sample
 void orc_evaluate_combos(const float *values, uint32_t n_bands, uint64_t n_samples, const double *ref_oklab,
                          double scale, const float *wl, const double *cie, const uint8_t *combos, uint32_t count,
                          uint64_t n_combos, const double *weights, const uint32_t *idx, uint64_t n_idx, int exact,
-                         double *out);
+                         int threads, double *out);
 
 #ifdef __cplusplus
 }

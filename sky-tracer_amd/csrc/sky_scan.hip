@@ -54,11 +54,13 @@ struct ScanArgs {
     const uint32_t *idx;     // samples to scan (null: 0..n-1)
     uint64_t n;              // number of samples scanned
     const uint8_t *combos;   // [n_combos][N]
-    const float *coef;       // [n_combos][N][3] LMS coefficients
+    const float *coef;       // [n_combos][N][3] XYZ coefficients
     uint64_t n_combos;
-    uint32_t R;
+    uint32_t R, B;
+    uint64_t n_total;
     QuadForm q;
     double *sum, *sq;
+    uint32_t *bad;           // set when a band or sample index is out of range
 };
 
 struct ErrArgs {
@@ -68,10 +70,19 @@ struct ErrArgs {
     uint64_t n;
     const uint8_t *combos;
     const float *coef;
-    uint32_t R, blocks_per_combo;
+    uint32_t R, B, blocks_per_combo;
     QuadForm q;
     float *err;              // [chunk][n]
+    uint32_t *bad;
 };
+
+// LMS coefficients of one channel: M1 c (the XYZ -> LMS matrix folded in)
+__device__ __forceinline__ void lms_coef(const float *c, float &l, float &m, float &s)
+{
+    l = fmaf((float)kM1[0][2], c[2], fmaf((float)kM1[0][1], c[1], (float)kM1[0][0] * c[0]));
+    m = fmaf((float)kM1[1][2], c[2], fmaf((float)kM1[1][1], c[1], (float)kM1[1][0] * c[0]));
+    s = fmaf((float)kM1[2][2], c[2], fmaf((float)kM1[2][1], c[1], (float)kM1[2][0] * c[0]));
+}
 
 __device__ __forceinline__ float cbrt_f(float x)
 {
@@ -101,10 +112,13 @@ __global__ __launch_bounds__(kWG) void k_scan(ScanArgs a)
 #pragma unroll
     for (int k = 0; k < N; ++k) {
         uint32_t b = live ? a.combos[c * N + k] : 0u;
+        if (b >= a.B) {
+            atomicOr(a.bad, 1u);
+            b = 0;
+        }
         addr[k] = b * kTileStride;
-        cl[k] = live ? a.coef[(c * N + k) * 3 + 0] : 0.0f;
-        cm[k] = live ? a.coef[(c * N + k) * 3 + 1] : 0.0f;
-        cs[k] = live ? a.coef[(c * N + k) * 3 + 2] : 0.0f;
+        cl[k] = cm[k] = cs[k] = 0.0f;
+        if (live) lms_coef(a.coef + (c * N + k) * 3, cl[k], cm[k], cs[k]);
     }
     const uint32_t ref = (a.R - 3) * kTileStride;
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -113,7 +127,11 @@ __global__ __launch_bounds__(kWG) void k_scan(ScanArgs a)
         const uint32_t nt = (uint32_t)min((uint64_t)kTile, a.n - base);
         __syncthreads();
         for (uint32_t r = wave; r < nt; r += kWG / 64) {
-            const uint64_t s = a.idx ? (uint64_t)a.idx[base + r] : base + r;
+            uint64_t s = a.idx ? (uint64_t)a.idx[base + r] : base + r;
+            if (s >= a.n_total) {
+                atomicOr(a.bad, 2u);
+                s = 0;
+            }
             const float *row = a.rows + s * a.R;
             for (uint32_t col = lane; col < a.R; col += 64) tile[col * kTileStride + r] = row[col];
         }
@@ -153,16 +171,25 @@ __global__ __launch_bounds__(256) void k_errors(ErrArgs a)
     const uint32_t combo = blockIdx.x / a.blocks_per_combo;
     const uint64_t i = (uint64_t)(blockIdx.x - combo * a.blocks_per_combo) * 256 + threadIdx.x;
     if (i >= a.n) return;
-    const uint64_t s = a.idx ? (uint64_t)a.idx[i] : i;
+    uint64_t s = a.idx ? (uint64_t)a.idx[i] : i;
+    if (s >= a.n_total) {
+        atomicOr(a.bad, 2u);
+        s = 0;
+    }
     float l = 0.0f, m = 0.0f, t = 0.0f;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        const uint32_t b = a.combos[(uint64_t)combo * N + k];
-        const float *cf = a.coef + ((uint64_t)combo * N + k) * 3;
+        uint32_t b = a.combos[(uint64_t)combo * N + k];
+        if (b >= a.B) {
+            atomicOr(a.bad, 1u);
+            b = 0;
+        }
+        float cl, cm, cs;
+        lms_coef(a.coef + ((uint64_t)combo * N + k) * 3, cl, cm, cs);
         float v = a.cols[b * a.n_total + s];
-        l = fmaf(v, cf[0], l);
-        m = fmaf(v, cf[1], m);
-        t = fmaf(v, cf[2], t);
+        l = fmaf(v, cl, l);
+        m = fmaf(v, cm, m);
+        t = fmaf(v, cs, t);
     }
     const uint32_t rb = a.R - 3;
     float e2 = dist_sq(l, m, t, a.cols[rb * a.n_total + s], a.cols[(rb + 1) * a.n_total + s],
@@ -376,7 +403,7 @@ struct sky_scan_dataset {
     uint32_t B = 0, R = 0;
     uint64_t n = 0;
     Buf rows, cols;  // [n][R] and [R][n]
-    Buf combos, coef, idx, sum, sq, err, stats;
+    Buf combos, coef, idx, sum, sq, err, stats, bad;
     QuadForm q{};
     ~sky_scan_dataset()
     {
@@ -388,14 +415,20 @@ struct sky_scan_dataset {
 
 namespace {
 
-// Validates the combo list and uploads combos, LMS coefficients and sample
+sky_status check_count(uint32_t count)
+{
+    if (count < 1 || count > SKY_SCAN_MAX_PRIMARIES)
+        return sky::set_error(SKY_ERR_INVALID_CONFIG, "combo size must be in 1..=8");
+    return SKY_OK;
+}
+
+// Validates the host combo list and uploads combos, coefficients and sample
 // indices; returns the number of samples scanned.
 sky_status upload_combos(sky_scan_dataset *ds, const uint8_t *combos, const float *coef, uint32_t count,
                          uint64_t n_combos, const uint32_t *samples, uint64_t n_samples, uint64_t *n_scan)
 {
     if (!ds || !combos || !coef) return sky::set_error(SKY_ERR_INVALID_CONFIG, "null dataset, combos or coefficients");
-    if (count < 1 || count > SKY_SCAN_MAX_PRIMARIES)
-        return sky::set_error(SKY_ERR_INVALID_CONFIG, "combo size must be in 1..=8");
+    if (check_count(count) != SKY_OK) return SKY_ERR_INVALID_CONFIG;
     for (uint64_t i = 0; i < n_combos * count; ++i)
         if (combos[i] >= ds->B) return sky::set_error(SKY_ERR_INVALID_CONFIG, "combo band index out of range");
     if (samples) {
@@ -404,22 +437,16 @@ sky_status upload_combos(sky_scan_dataset *ds, const uint8_t *combos, const floa
             if (samples[i] >= ds->n) return sky::set_error(SKY_ERR_INVALID_CONFIG, "sample index out of range");
     }
     *n_scan = samples ? n_samples : ds->n;
-    std::vector<float> lms(n_combos * count * 3);
-    for (uint64_t i = 0; i < n_combos * count; ++i)
-        for (int r = 0; r < 3; ++r)
-            lms[3 * i + r] = (float)(kM1[r][0] * coef[3 * i] + kM1[r][1] * coef[3 * i + 1] + kM1[r][2] * coef[3 * i + 2]);
+    const size_t coef_bytes = n_combos * count * 3 * sizeof(float);
     SCAN_TRY(ds->combos.ensure(n_combos * count), "alloc combos");
-    SCAN_TRY(ds->coef.ensure(lms.size() * sizeof(float)), "alloc coefficients");
+    SCAN_TRY(ds->coef.ensure(coef_bytes), "alloc coefficients");
     SCAN_TRY(hipMemcpyAsync(ds->combos.p, combos, n_combos * count, hipMemcpyHostToDevice, ds->stream), "upload combos");
-    SCAN_TRY(hipMemcpyAsync(ds->coef.p, lms.data(), lms.size() * sizeof(float), hipMemcpyHostToDevice, ds->stream),
-             "upload coefficients");
+    SCAN_TRY(hipMemcpyAsync(ds->coef.p, coef, coef_bytes, hipMemcpyHostToDevice, ds->stream), "upload coefficients");
     if (samples) {
         SCAN_TRY(ds->idx.ensure(n_samples * sizeof(uint32_t)), "alloc sample list");
         SCAN_TRY(hipMemcpyAsync(ds->idx.p, samples, n_samples * sizeof(uint32_t), hipMemcpyHostToDevice, ds->stream),
                  "upload sample list");
     }
-    // the staging vector must outlive the copy
-    SCAN_TRY(hipStreamSynchronize(ds->stream), "synchronize");
     return SKY_OK;
 }
 
@@ -436,9 +463,49 @@ void launch_errors(const ErrArgs &a, uint32_t chunk, hipStream_t st)
     hipLaunchKernelGGL(k_errors<N>, dim3(chunk * a.blocks_per_combo), dim3(256), 0, st, a);
 }
 
-double elapsed_ms(std::chrono::steady_clock::time_point t0)
+// Clears the index-error flag, records ev0 and launches the screening kernel
+// on `st` (device pointers throughout), then records ev1.
+sky_status launch_screen(sky_scan_dataset *ds, const uint8_t *d_combos, const float *d_coef, uint32_t count,
+                         uint64_t n_combos, const uint32_t *d_idx, uint64_t n, double *d_sum, double *d_sq,
+                         hipStream_t st)
 {
-    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if ((n_combos + kWG - 1) / kWG >= (1ull << 31)) return sky::set_error(SKY_ERR_INVALID_CONFIG, "too many combos");
+    SCAN_TRY(hipMemsetAsync(ds->bad.p, 0, sizeof(uint32_t), st), "clear flag");
+    ScanArgs a{(const float *)ds->rows.p, d_idx, n, d_combos, d_coef, n_combos, ds->R, ds->B, ds->n, ds->q,
+               d_sum, d_sq, (uint32_t *)ds->bad.p};
+    const size_t lds = (size_t)ds->R * kTileStride * sizeof(float);
+    SCAN_TRY(hipEventRecord(ds->ev0, st), "record event");
+    switch (count) {
+    case 1: launch_scan<1>(a, lds, st); break;
+    case 2: launch_scan<2>(a, lds, st); break;
+    case 3: launch_scan<3>(a, lds, st); break;
+    case 4: launch_scan<4>(a, lds, st); break;
+    case 5: launch_scan<5>(a, lds, st); break;
+    case 6: launch_scan<6>(a, lds, st); break;
+    case 7: launch_scan<7>(a, lds, st); break;
+    default: launch_scan<8>(a, lds, st); break;
+    }
+    SCAN_TRY(hipGetLastError(), "launch scan");
+    SCAN_TRY(hipEventRecord(ds->ev1, st), "record event");
+    return SKY_OK;
+}
+
+// After the stream has synchronised: the index-error flag and the timing.
+sky_status finish(sky_scan_dataset *ds, uint64_t evaluations, std::chrono::steady_clock::time_point t0,
+                  sky_scan_stats *stats)
+{
+    uint32_t bad = 0;
+    SCAN_TRY(hipMemcpy(&bad, ds->bad.p, sizeof bad, hipMemcpyDeviceToHost), "read flag");
+    if (bad & 1u) return sky::set_error(SKY_ERR_INVALID_CONFIG, "combo band index out of range");
+    if (bad & 2u) return sky::set_error(SKY_ERR_INVALID_CONFIG, "sample index out of range");
+    if (stats) {
+        float ms = 0.0f;
+        SCAN_TRY(hipEventElapsedTime(&ms, ds->ev0, ds->ev1), "event time");
+        stats->kernel_ms = ms;
+        stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        stats->evaluations = evaluations;
+    }
+    return SKY_OK;
 }
 
 }  // namespace
@@ -484,6 +551,7 @@ sky_status sky_scan_dataset_create(int device, const float *band_values, uint32_
             rows[s * R + n_bands + r] = (float)(inv[r][0] * lab[0] + inv[r][1] * lab[1] + inv[r][2] * lab[2]);
         for (uint64_t c = 0; c < R; ++c) cols[c * n_samples + s] = rows[s * R + c];
     }
+    if ((e = ds->bad.ensure(sizeof(uint32_t))) != hipSuccess) return bail(hip_err(e, "alloc flag"));
     if ((e = ds->rows.ensure(rows.size() * sizeof(float))) != hipSuccess) return bail(hip_err(e, "alloc samples"));
     if ((e = ds->cols.ensure(cols.size() * sizeof(float))) != hipSuccess) return bail(hip_err(e, "alloc samples"));
     e = hipMemcpy(ds->rows.p, rows.data(), rows.size() * sizeof(float), hipMemcpyHostToDevice);
@@ -513,34 +581,33 @@ sky_status sky_scan_combos(sky_scan_dataset *ds, const uint8_t *combos, const fl
     if (st != SKY_OK) return st;
     SCAN_TRY(ds->sum.ensure(n_combos * sizeof(double)), "alloc sums");
     SCAN_TRY(ds->sq.ensure(n_combos * sizeof(double)), "alloc sums");
-    ScanArgs a{(const float *)ds->rows.p, samples ? (const uint32_t *)ds->idx.p : nullptr, n,
-               (const uint8_t *)ds->combos.p, (const float *)ds->coef.p, n_combos, ds->R, ds->q,
-               (double *)ds->sum.p, (double *)ds->sq.p};
-    const size_t lds = (size_t)ds->R * kTileStride * sizeof(float);
-    SCAN_TRY(hipEventRecord(ds->ev0, ds->stream), "record event");
-    switch (count) {
-    case 1: launch_scan<1>(a, lds, ds->stream); break;
-    case 2: launch_scan<2>(a, lds, ds->stream); break;
-    case 3: launch_scan<3>(a, lds, ds->stream); break;
-    case 4: launch_scan<4>(a, lds, ds->stream); break;
-    case 5: launch_scan<5>(a, lds, ds->stream); break;
-    case 6: launch_scan<6>(a, lds, ds->stream); break;
-    case 7: launch_scan<7>(a, lds, ds->stream); break;
-    default: launch_scan<8>(a, lds, ds->stream); break;
-    }
-    SCAN_TRY(hipGetLastError(), "launch scan");
-    SCAN_TRY(hipEventRecord(ds->ev1, ds->stream), "record event");
+    st = launch_screen(ds, (const uint8_t *)ds->combos.p, (const float *)ds->coef.p, count, n_combos,
+                       samples ? (const uint32_t *)ds->idx.p : nullptr, n, (double *)ds->sum.p, (double *)ds->sq.p,
+                       ds->stream);
+    if (st != SKY_OK) return st;
     SCAN_TRY(hipMemcpyAsync(sum, ds->sum.p, n_combos * sizeof(double), hipMemcpyDeviceToHost, ds->stream), "copy sums");
     SCAN_TRY(hipMemcpyAsync(sum_sq, ds->sq.p, n_combos * sizeof(double), hipMemcpyDeviceToHost, ds->stream), "copy sums");
     SCAN_TRY(hipStreamSynchronize(ds->stream), "synchronize");
-    if (stats) {
-        float ms = 0.0f;
-        SCAN_TRY(hipEventElapsedTime(&ms, ds->ev0, ds->ev1), "event time");
-        stats->kernel_ms = ms;
-        stats->total_ms = elapsed_ms(t0);
-        stats->evaluations = n_combos * n;
-    }
-    return SKY_OK;
+    return finish(ds, n_combos * n, t0, stats);
+}
+
+sky_status sky_scan_combos_device(sky_scan_dataset *ds, const uint8_t *d_combos, const float *d_coef, uint32_t count,
+                                  uint64_t n_combos, const uint32_t *d_samples, uint64_t n_samples, double *d_sum,
+                                  double *d_sum_sq, void *stream, sky_scan_stats *stats)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    if (!ds || !d_combos || !d_coef || !d_sum || !d_sum_sq)
+        return sky::set_error(SKY_ERR_INVALID_CONFIG, "null dataset, combos, coefficients or output");
+    if (check_count(count) != SKY_OK) return SKY_ERR_INVALID_CONFIG;
+    if (d_samples && n_samples == 0) return sky::set_error(SKY_ERR_INVALID_CONFIG, "empty sample list");
+    if (n_combos == 0) return SKY_OK;
+    SCAN_TRY(hipSetDevice(ds->device), "set device");
+    hipStream_t st = stream ? (hipStream_t)stream : ds->stream;
+    const uint64_t n = d_samples ? n_samples : ds->n;
+    sky_status s = launch_screen(ds, d_combos, d_coef, count, n_combos, d_samples, n, d_sum, d_sum_sq, st);
+    if (s != SKY_OK) return s;
+    SCAN_TRY(hipStreamSynchronize(st), "synchronize");
+    return finish(ds, n_combos * n, t0, stats);
 }
 
 sky_status sky_refine_combos(sky_scan_dataset *ds, const uint8_t *combos, const float *coef, uint32_t count,
@@ -561,12 +628,13 @@ sky_status sky_refine_combos(sky_scan_dataset *ds, const uint8_t *combos, const 
     SCAN_TRY(ds->stats.ensure(n_combos * sizeof(sky_combo_stats)), "alloc statistics");
     const uint32_t bpc = (uint32_t)((n + 255) / 256);
     if ((uint64_t)bpc * chunk >= (1ull << 31)) return sky::set_error(SKY_ERR_INVALID_CONFIG, "refinement too large");
+    SCAN_TRY(hipMemsetAsync(ds->bad.p, 0, sizeof(uint32_t), ds->stream), "clear flag");
     SCAN_TRY(hipEventRecord(ds->ev0, ds->stream), "record event");
     for (uint64_t c0 = 0; c0 < n_combos; c0 += chunk) {
         const uint32_t nc = (uint32_t)std::min<uint64_t>(chunk, n_combos - c0);
         ErrArgs a{(const float *)ds->cols.p, ds->n, samples ? (const uint32_t *)ds->idx.p : nullptr, n,
-                  (const uint8_t *)ds->combos.p + c0 * count, (const float *)ds->coef.p + c0 * count * 3, ds->R, bpc,
-                  ds->q, (float *)ds->err.p};
+                  (const uint8_t *)ds->combos.p + c0 * count, (const float *)ds->coef.p + c0 * count * 3, ds->R,
+                  ds->B, bpc, ds->q, (float *)ds->err.p, (uint32_t *)ds->bad.p};
         switch (count) {
         case 1: launch_errors<1>(a, nc, ds->stream); break;
         case 2: launch_errors<2>(a, nc, ds->stream); break;
@@ -586,14 +654,7 @@ sky_status sky_refine_combos(sky_scan_dataset *ds, const uint8_t *combos, const 
     SCAN_TRY(hipMemcpyAsync(out, ds->stats.p, n_combos * sizeof(sky_combo_stats), hipMemcpyDeviceToHost, ds->stream),
              "copy statistics");
     SCAN_TRY(hipStreamSynchronize(ds->stream), "synchronize");
-    if (stats) {
-        float ms = 0.0f;
-        SCAN_TRY(hipEventElapsedTime(&ms, ds->ev0, ds->ev1), "event time");
-        stats->kernel_ms = ms;
-        stats->total_ms = elapsed_ms(t0);
-        stats->evaluations = n_combos * n;
-    }
-    return SKY_OK;
+    return finish(ds, n_combos * n, t0, stats);
 }
 
 }  // extern "C"

@@ -188,6 +188,8 @@ def lib() -> C.CDLL:
         "sky_scan_dataset_destroy": ([ctxp], None),
         "sky_scan_combos": ([ctxp, C.c_void_p, fp, C.c_uint32, C.c_uint64, C.c_void_p, C.c_uint64, dp, dp,
                              C.POINTER(ScanStats)], C.c_int),
+        "sky_scan_combos_device": ([ctxp, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint64, C.c_void_p, C.c_uint64,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(ScanStats)], C.c_int),
         "sky_refine_combos": ([ctxp, C.c_void_p, fp, C.c_uint32, C.c_uint64, C.c_void_p, C.c_uint64,
                                C.POINTER(ComboStats), C.POINTER(ScanStats)], C.c_int),
         "sky_band_widths": ([fp, C.c_uint32, dp], C.c_int),

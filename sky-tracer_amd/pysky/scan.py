@@ -136,6 +136,21 @@ class ScanDataset:
                                       _dp(s), _dp(q), C.byref(st)))
         return s, q, st
 
+    def scan_device(self, combos, coef, out_sum, out_sq, samples=None, stream=None):
+        """Screening sums with every operand a device tensor (uint8 [n, count],
+        float32 [n, count, 3], float64 [n] outputs, optional uint32 samples),
+        ordered on torch's current stream unless `stream` is given."""
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream(combos.device).cuda_stream
+        n, count = combos.shape
+        st = L.ScanStats()
+        check(L.lib().sky_scan_combos_device(self._h, combos.data_ptr(), coef.data_ptr(), count, n,
+                                             None if samples is None else samples.data_ptr(),
+                                             0 if samples is None else samples.numel(), out_sum.data_ptr(),
+                                             out_sq.data_ptr(), stream, C.byref(st)))
+        return st
+
     def refine(self, combos, coef, samples: Optional[np.ndarray] = None):
         """Per combo mean, rmse, p50, p90, p95, p99, max (dict of arrays) and stats."""
         cb, cf, sm = self._args(combos, coef, samples)

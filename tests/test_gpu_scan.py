@@ -149,3 +149,33 @@ def test_invalid_requests(data):
         ds.scan(np.array([[0, 1]], np.uint8), coef, np.array([0, 4096], np.uint32))
     with pytest.raises(InvalidConfig):
         ds.scan(np.zeros((1, 9), np.uint8), np.ones((1, 9, 3), np.float32))
+
+
+def test_device_pointer_variant(data):
+    import torch
+    v, cie, ref, scale, ds = data
+    combos = _combos(6, 700, 6)
+    _, coef = _coef(combos, scan.VORONOI, cie, scale)
+    idx = scan.screening_indices(v.shape[0], 777)
+    s, q, _ = ds.scan(combos, coef, idx)
+    dev = torch.device("cuda:0")
+    tc = torch.from_numpy(combos).to(dev)
+    tf = torch.from_numpy(coef).to(dev)
+    ti = torch.from_numpy(idx.astype(np.int32)).to(dev)
+    ts = torch.zeros(700, dtype=torch.float64, device=dev)
+    tq = torch.zeros(700, dtype=torch.float64, device=dev)
+    st = ds.scan_device(tc, tf, ts, tq, samples=ti)
+    assert st.evaluations == 700 * 777
+    assert np.array_equal(ts.cpu().numpy(), s) and np.array_equal(tq.cpu().numpy(), q)
+    # out-of-range indices are clamped on the device and reported
+    bad = tc.clone()
+    bad[3, 2] = 41
+    with pytest.raises(InvalidConfig):
+        ds.scan_device(bad, tf, ts, tq)
+    bad_i = ti.clone()
+    bad_i[5] = v.shape[0]
+    with pytest.raises(InvalidConfig):
+        ds.scan_device(tc, tf, ts, tq, samples=bad_i)
+    # and the dataset still works afterwards
+    ds.scan_device(tc, tf, ts, tq, samples=ti)
+    assert np.array_equal(ts.cpu().numpy(), s)
