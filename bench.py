@@ -39,6 +39,9 @@ PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 (packed vector FMA); unpacked vector iss
 # [pixel][band] float2 partial per pixel group and reads only the scene tables, so the figure
 # does not depend on spp at the tuned work split.
 TRAFFIC_BYTES = {"c2": (2.0 * 1853.5 + 278208.1) * 1024}
+# Same for the combo scan of all 6-band combos x 65536 samples:
+# profiles/k_scan_6band_combos_x_65536_samples_counters.md.
+SCAN_TRAFFIC_BYTES = {6: (2.0 * 5189454.0 + 70256.1) * 1024}
 
 
 def make_scene(w):
@@ -386,7 +389,7 @@ def bench_scan(args):
                                f"65536 of 655360 sky-view LUT samples ({args.spp or 64}-spp bake), one launch",
                    "parallelism": "1 GPU"},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP32_TFLOPS, 5), "traffic": None,
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 5), "traffic": SCAN_TRAFFIC_BYTES.get(P),
                      "flops_per_eval": f_eval, "kernel": f"k_scan<{P}>", "kernel_ms_avg": round(avg_s * 1e3, 2)},
         "cpu_baseline": cpu,
         "best_screened_combo_nm": [float(wl[b]) for b in best],
