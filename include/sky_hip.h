@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define SKY_HIP_ABI_VERSION 2
+#define SKY_HIP_ABI_VERSION 3
 
 /* RenderError (integrator.rs:19-26) as status codes. */
 typedef enum sky_status {
@@ -53,6 +53,29 @@ typedef struct sky_planet {
     double atmosphere_radius_km;
 } sky_planet;
 
+/* Tabulated medium: the reference's SceneData profiles and optics
+ * (crates/sky-core/src/data.rs:53-152, atmosphere.rs:40-58, medium.rs:28-57):
+ *   sigma_R = air_cm3(h) * rayleigh_cross_section_m2 * 1e9       (km^-1)
+ *   sigma_O = ozone_cm3(h) * ozone_cross_section_cm2 * 1e5
+ *   sigma_As = sum_s mass_s(h) * scattering_s,  sigma_Aa likewise
+ * with every profile interpolated linearly in altitude, clamped to its end
+ * points (medium.rs:108-155).  Profiles are given in ascending altitude.  The
+ * aerosol phase is Henyey-Greenstein with sky_atmosphere.aerosol_g (the
+ * reference's tabulated Mie phase, data/mie_phase.csv, is absent from its
+ * checkout). */
+#define SKY_AEROSOL_SPECIES 4 /* inso, waso, soot, suso (atmosphere.rs:6-8) */
+typedef struct sky_medium_tables {
+    uint32_t atmosphere_points;    /* rows of atmosphere_profile.csv (>= 1) */
+    uint32_t aerosol_points;       /* rows of aerosol_profile.csv (>= 1)    */
+    const double *atmosphere;      /* [atmosphere_points][3]: altitude_km, air_cm3, ozone_cm3 */
+    const double *aerosol;         /* [aerosol_points][5]: altitude_km, mass_g_m3 of the 4 species */
+    const double *aerosol_optics;  /* [count][4][2]: scattering, absorption (km^-1 per g/m^3) */
+    const double *ozone_cross_section_cm2;   /* [count] */
+    const double *rayleigh_cross_section_m2; /* [count]; NULL: 5.8e-31 (550/lambda)^4 (medium.rs:104-106) */
+    const double *solar_irradiance_w_m2;     /* [count] band irradiance, radiance = E / sun solid angle
+                                                (data.rs:30-33); NULL: Planck at sun_temperature_k */
+} sky_medium_tables;
+
 /* Analytic replacement of the tabulated SceneData medium
  * (crates/sky-core/src/atmosphere.rs:121-133, medium.rs:28-55):
  *   sigma_R = rayleigh_sca_550 (550/l)^4 exp(-h/H_R)
@@ -75,6 +98,10 @@ typedef struct sky_atmosphere {
     double ground_albedo;          /* 0.3       */
     double sun_temperature_k;      /* 5778      */
     double sun_angular_radius_rad; /* 0.2675 deg */
+    /* NULL: the analytic profiles above.  Otherwise the tabulated medium
+     * (count = the wavelength set's count); only aerosol_g, ground_albedo,
+     * sun_temperature_k and sun_angular_radius_rad are then used. */
+    const sky_medium_tables *tables;
 } sky_atmosphere;
 
 /* Wavelength set (replaces the fixed 41 bands, spectrum.rs:5).  weight_nm

@@ -64,6 +64,21 @@ class _Params(C.Structure):
         ("lambda_nm", C.POINTER(C.c_double)),
         ("weight_nm", C.POINTER(C.c_double)),
         ("shell_growth", C.c_double),
+        ("tables", C.c_void_p),
+    ]
+
+
+class _Tables(C.Structure):
+    """orc_tables: the reference's tabulated SceneData (data.rs:53-152)."""
+    _fields_ = [
+        ("atmosphere_points", C.c_uint32),
+        ("aerosol_points", C.c_uint32),
+        ("atmosphere", C.POINTER(C.c_double)),
+        ("aerosol", C.POINTER(C.c_double)),
+        ("aerosol_optics", C.POINTER(C.c_double)),
+        ("ozone_cross_section_cm2", C.POINTER(C.c_double)),
+        ("rayleigh_cross_section_m2", C.POINTER(C.c_double)),
+        ("solar_irradiance_w_m2", C.POINTER(C.c_double)),
     ]
 
 
@@ -125,6 +140,9 @@ def lib():
             fn.argtypes = [P, dp, dp, C.c_double, dp, C.POINTER(C.c_int), C.c_int]
         L.orc_optical_depth.argtypes = [P, C.POINTER(_Lambda), dp, dp, C.c_double]
         L.orc_optical_depth.restype = C.c_double
+        L.orc_optical_depth_band.argtypes = [P, C.c_uint32, dp, dp, C.c_double]
+        L.orc_optical_depth_band.restype = C.c_double
+        L.orc_coefficients_band.argtypes = [P, C.c_uint32, C.c_double, dp, dp, dp, dp]
         L.orc_camera_dir.argtypes = [P, C.c_double, C.c_double, dp]
         for s in ("f32", "f64"):
             fn = getattr(L, f"orc_render_{s}")
@@ -212,13 +230,31 @@ class Params:
     lambda_nm: Sequence[float] = field(default_factory=lambda: list(midpoint_grid(16)))
     weight_nm: Optional[Sequence[float]] = None
     shell_growth: float = 1.0
+    # Tabulated medium: any object with the arrays atmosphere [n, 3] (altitude,
+    # air_cm3, ozone_cm3), aerosol [n, 5], aerosol_optics [n_lambda, 4, 2],
+    # ozone_cross_section_cm2 [n_lambda] and optional rayleigh_cross_section_m2,
+    # solar_irradiance_w_m2 [n_lambda].  None: the analytic profiles.
+    tables: Optional[object] = None
 
     def to_c(self):
         p = _Params()
         for f in fields(self):
-            if f.name in ("lambda_nm", "weight_nm"):
+            if f.name in ("lambda_nm", "weight_nm", "tables"):
                 continue
             setattr(p, f.name, getattr(self, f.name))
+        keep = []
+        if self.tables is not None:
+            t = self.tables
+            arrs = {k: np.ascontiguousarray(getattr(t, k), dtype=np.float64)
+                    for k in ("atmosphere", "aerosol", "aerosol_optics", "ozone_cross_section_cm2")}
+            opt = {k: (None if getattr(t, k, None) is None
+                       else np.ascontiguousarray(getattr(t, k), dtype=np.float64))
+                   for k in ("rayleigh_cross_section_m2", "solar_irradiance_w_m2")}
+            tc = _Tables(atmosphere_points=arrs["atmosphere"].shape[0], aerosol_points=arrs["aerosol"].shape[0])
+            for k, a in {**arrs, **opt}.items():
+                setattr(tc, k, None if a is None else _dp(a))
+            p.tables = C.cast(C.pointer(tc), C.c_void_p)
+            keep += [tc, arrs, opt]
         lam = np.ascontiguousarray(self.lambda_nm, dtype=np.float64)
         if self.weight_nm is None:
             n = len(lam)
@@ -229,7 +265,7 @@ class Params:
         p.lambda_nm = _dp(lam)
         p.weight_nm = _dp(w)
         # keep the arrays alive with the struct
-        p._keep = (lam, w)
+        p._keep = (lam, w, keep)
         return p
 
     @property
@@ -286,9 +322,10 @@ def _lambda_struct(p: Params, band: int):
 
 
 def coefficients(p: Params, band: int, h: float) -> np.ndarray:
-    cp, lam = _lambda_struct(p, band)
+    """[sigma_R, sigma_As, sigma_O, sigma_Aa] (km^-1) of band at altitude h."""
+    cp = p.to_c()
     out = [C.c_double() for _ in range(4)]
-    lib().orc_coefficients(C.byref(cp), C.byref(lam), h, *[C.byref(o) for o in out])
+    lib().orc_coefficients_band(C.byref(cp), band, h, *[C.byref(o) for o in out])
     return np.array([o.value for o in out])
 
 
@@ -339,10 +376,10 @@ def segments(p: Params, o, d, t_max: float, reference: bool):
 
 
 def optical_depth(p: Params, band: int, o, d, t_max: float) -> float:
-    cp, lam = _lambda_struct(p, band)
+    cp = p.to_c()
     o = np.ascontiguousarray(o, dtype=np.float64)
     d = np.ascontiguousarray(d, dtype=np.float64)
-    return float(lib().orc_optical_depth(C.byref(cp), C.byref(lam), _dp(o), _dp(d), t_max))
+    return float(lib().orc_optical_depth_band(C.byref(cp), band, _dp(o), _dp(d), t_max))
 
 
 def camera_dir(p: Params, u: float, v: float) -> np.ndarray:

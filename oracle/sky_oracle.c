@@ -135,6 +135,49 @@ void orc_default_params(orc_params *p)
 /* Per-wavelength coefficients of the analytic medium (data.rs:26-33 and
  * medium.rs:104-106 restated): Rayleigh beta*(550/l)^4, ozone Gaussian band,
  * aerosol Angstrom law; all converted from m^-1 to km^-1. */
+/* bracket + lerp of one profile column (medium.rs:108-155): partition_point
+ * of z < x, clamped to the end points, a (1 - t) + b t. */
+static double profile_lerp(const double *rows, uint32_t n, uint32_t stride, uint32_t col, double x)
+{
+    uint32_t hi = 0;
+    while (hi < n && rows[hi * stride] < x) ++hi;
+    if (hi == 0) return rows[col];
+    if (hi >= n) return rows[(n - 1) * stride + col];
+    const double *lo = rows + (hi - 1) * stride, *up = rows + hi * stride;
+    double t = up[0] > lo[0] ? (x - lo[0]) / (up[0] - lo[0]) : 0.0;
+    if (t < 0.0) t = 0.0;
+    if (t > 1.0) t = 1.0;
+    return lo[col] * (1.0 - t) + up[col] * t;
+}
+
+static double rayleigh_xs(const orc_params *p, uint32_t b)
+{
+    const orc_tables *t = p->tables;
+    if (t->rayleigh_cross_section_m2) return t->rayleigh_cross_section_m2[b];
+    double r = 550.0 / p->lambda_nm[b];
+    return 5.8e-31 * r * r * r * r; /* medium.rs:104-106 */
+}
+
+/* coefficients_at (medium.rs:28-57) for band b at altitude h (0 <= h <= top). */
+static void table_coefficients(const orc_params *p, uint32_t b, double h, double *sr, double *sas, double *so,
+                               double *saa)
+{
+    const orc_tables *t = p->tables;
+    double air = profile_lerp(t->atmosphere, t->atmosphere_points, 3, 1, h);
+    double o3 = profile_lerp(t->atmosphere, t->atmosphere_points, 3, 2, h);
+    *sr = air * rayleigh_xs(p, b) * 1.0e9;
+    *so = o3 * t->ozone_cross_section_cm2[b] * 1.0e5;
+    if (*sr < 0.0) *sr = 0.0;
+    if (*so < 0.0) *so = 0.0;
+    *sas = *saa = 0.0;
+    for (int s = 0; s < ORC_SPECIES; ++s) {
+        double m = profile_lerp(t->aerosol, t->aerosol_points, 5, 1 + s, h);
+        if (m < 0.0) m = 0.0;
+        *sas += m * t->aerosol_optics[(b * ORC_SPECIES + s) * 2];
+        *saa += m * t->aerosol_optics[(b * ORC_SPECIES + s) * 2 + 1];
+    }
+}
+
 int orc_lambda_table(const orc_params *p, orc_lambda *out)
 {
     if (p->n_lambda == 0 || p->n_lambda > ORC_MAX_LAMBDA || !p->lambda_nm) return 1;
@@ -148,6 +191,15 @@ int orc_lambda_table(const orc_params *p, orc_lambda *out)
         out[i].beta_as = p->aerosol_sca_550_m_inv * 1000.0 * ang;
         out[i].beta_aa = p->aerosol_abs_550_m_inv * 1000.0 * ang;
         out[i].sun_radiance = orc_planck_radiance(l, p->sun_temperature_k);
+        if (p->tables) {
+            /* sea-level coefficients (for inspection; the tracer reads the profiles) */
+            table_coefficients(p, i, 0.0, &out[i].beta_r, &out[i].beta_as, &out[i].beta_o, &out[i].beta_aa);
+            if (p->tables->solar_irradiance_w_m2) {
+                /* solar_irradiance / solid angle 2 pi (1 - cos r) (data.rs:30-33, atmosphere.rs:26) */
+                double hs = sin(0.5 * p->sun_angular_radius_rad);
+                out[i].sun_radiance = p->tables->solar_irradiance_w_m2[i] / (4.0 * ORC_PI * hs * hs);
+            }
+        }
         double cmf[3];
         orc_cmf_xyz(l, cmf);
         double w = p->weight_nm ? p->weight_nm[i] : 1.0;
@@ -170,6 +222,23 @@ void orc_sun_direction(const orc_params *p, double dir[3])
 
 /* medium.rs:28-55 for the analytic profiles; zero outside [0, top]
  * (medium.rs:30-34, sky_trace.wgsl:358-360). */
+void orc_coefficients_band(const orc_params *p, uint32_t band, double h, double *sr, double *sas, double *so,
+                           double *saa)
+{
+    double top = p->atmosphere_radius_km - p->ground_radius_km;
+    if (h < 0.0 || h > top) {
+        *sr = *sas = *so = *saa = 0.0;
+        return;
+    }
+    if (p->tables) {
+        table_coefficients(p, band, h, sr, sas, so, saa);
+        return;
+    }
+    orc_lambda lam[ORC_MAX_LAMBDA];
+    orc_lambda_table(p, lam);
+    orc_coefficients(p, &lam[band], h, sr, sas, so, saa);
+}
+
 void orc_coefficients(const orc_params *p, const orc_lambda *lam, double h, double *sr, double *sas,
                       double *so, double *saa)
 {
@@ -219,6 +288,36 @@ int orc_majorant_grid(const orc_params *p, double *out)
     if (orc_lambda_table(p, lam)) return 1;
     uint32_t L = p->majorant_layers;
     if (L == 0 || L > ORC_MAX_LAYERS) return 1;
+    if (p->tables) {
+        /* compute_majorant_grid (medium.rs:63-102): max over the shell's ends,
+         * middle and the profile altitudes inside it, times the margin */
+        const orc_tables *t = p->tables;
+        for (uint32_t b = 0; b < p->n_lambda; ++b)
+            for (uint32_t k = 0; k < L; ++k) {
+                double lo = orc_shell_height(p, (int)k), hi = orc_shell_height(p, (int)k + 1);
+                double probes[3] = {lo, hi, 0.5 * (lo + hi)}, m = 0.0, c[4];
+                for (int i = 0; i < 3; ++i) {
+                    orc_coefficients_band(p, b, probes[i], &c[0], &c[1], &c[2], &c[3]);
+                    double e = c[0] + c[1] + c[2] + c[3];
+                    if (e > m) m = e;
+                }
+                for (int pass = 0; pass < 2; ++pass) {
+                    uint32_t n = pass ? t->aerosol_points : t->atmosphere_points;
+                    const double *rows = pass ? t->aerosol : t->atmosphere;
+                    uint32_t stride = pass ? 5 : 3;
+                    for (uint32_t i = 0; i < n; ++i) {
+                        double z = rows[i * stride];
+                        if (!(z > lo && z < hi)) continue;
+                        orc_coefficients_band(p, b, z, &c[0], &c[1], &c[2], &c[3]);
+                        double e = c[0] + c[1] + c[2] + c[3];
+                        if (e > m) m = e;
+                    }
+                }
+                m *= p->majorant_margin;
+                out[b * L + k] = m > 1e-8 ? m : 1e-8;
+            }
+        return 0;
+    }
     for (uint32_t b = 0; b < p->n_lambda; ++b) {
         for (uint32_t k = 0; k < L; ++k) {
             double lo = orc_shell_height(p, (int)k), hi = orc_shell_height(p, (int)k + 1);
@@ -437,12 +536,19 @@ static void gl16_init(void)
     done = 1;
 }
 
+/* Band of the current optical-depth integral (tabulated medium only). */
+static _Thread_local uint32_t od_band;
+
 static double sigma_t_at(const orc_params *p, const orc_lambda *lam, const double o[3],
                          const double d[3], double t)
 {
     double pos[3], sr, sas, so, saa;
     ray_at(o, d, t, pos);
-    orc_coefficients(p, lam, sqrt(dot3(pos, pos)) - p->ground_radius_km, &sr, &sas, &so, &saa);
+    double h = sqrt(dot3(pos, pos)) - p->ground_radius_km;
+    if (p->tables)
+        orc_coefficients_band(p, od_band, h, &sr, &sas, &so, &saa);
+    else
+        orc_coefficients(p, lam, h, &sr, &sas, &so, &saa);
     return sr + sas + so + saa;
 }
 
@@ -466,22 +572,41 @@ static int cmp_double(const void *a, const void *b)
  * every crossing of the majorant shells, of the ozone tent's kinks and of
  * the perigee, then each piece is integrated by 16-point Gauss-Legendre
  * (split further so that no piece exceeds 8 km). */
+double orc_optical_depth_band(const orc_params *p, uint32_t band, const double o[3], const double d[3],
+                              double t_max)
+{
+    orc_lambda lam[ORC_MAX_LAMBDA];
+    if (band >= p->n_lambda || orc_lambda_table(p, lam)) return NAN;
+    od_band = band;
+    return orc_optical_depth(p, &lam[band], o, d, t_max);
+}
+
 double orc_optical_depth(const orc_params *p, const orc_lambda *lam, const double o[3],
                          const double d[3], double t_max)
 {
     gl16_init();
     double R = p->ground_radius_km;
     double b = dot3(o, d), q = dot3(o, o) - b * b - R * R;
-    double cuts[1024];
+    double cuts[2048];
     int n = 0;
     cuts[n++] = 0.0;
     cuts[n++] = t_max;
     if (-b > 0.0 && -b < t_max) cuts[n++] = -b;
-    double kinks[3] = {p->ozone_center_km - p->ozone_half_width_km, p->ozone_center_km,
-                       p->ozone_center_km + p->ozone_half_width_km};
+    /* kinks of the integrand: the ozone tent, or every profile altitude */
+    double kinks[512];
+    int nk = 0;
+    if (p->tables) {
+        const orc_tables *t = p->tables;
+        for (uint32_t i = 0; i < t->atmosphere_points && nk < 256; ++i) kinks[nk++] = t->atmosphere[3 * i];
+        for (uint32_t i = 0; i < t->aerosol_points && nk < 512; ++i) kinks[nk++] = t->aerosol[5 * i];
+    } else {
+        kinks[nk++] = p->ozone_center_km - p->ozone_half_width_km;
+        kinks[nk++] = p->ozone_center_km;
+        kinks[nk++] = p->ozone_center_km + p->ozone_half_width_km;
+    }
     double top = p->atmosphere_radius_km - R;
     int L = (int)p->majorant_layers;
-    for (int k = 0; k <= L + 3 && n < 1000; ++k) {
+    for (int k = 0; k <= L + nk && n < 2000; ++k) {
         double hk = k <= L ? k * top / L : kinks[k - L - 1];
         double disc = hk * (2.0 * R + hk) - q;
         if (disc <= 0.0) continue;
@@ -581,6 +706,12 @@ static int validate(const orc_params *p)
     if (!(p->atmosphere_radius_km > p->ground_radius_km) || !(p->ground_radius_km > 0)) return 1;
     for (uint32_t i = 0; i < p->n_lambda; ++i)
         if (!(p->lambda_nm[i] > 0)) return 1;
+    if (p->tables) {
+        const orc_tables *t = p->tables;
+        if (!t->atmosphere || !t->aerosol || !t->aerosol_optics || !t->ozone_cross_section_cm2 ||
+            t->atmosphere_points == 0 || t->aerosol_points == 0)
+            return 1;
+    }
     return 0;
 }
 

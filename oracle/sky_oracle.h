@@ -50,6 +50,20 @@ typedef struct orc_counters {
     uint64_t watchdog;         /* paths that tripped a watchdog             */
 } orc_counters;
 
+/* The reference's tabulated SceneData (data.rs:53-152): profiles in
+ * ascending altitude, per-wavelength optics.  NULL in orc_params: the
+ * analytic profiles. */
+#define ORC_SPECIES 4 /* inso, waso, soot, suso (atmosphere.rs:6-8) */
+typedef struct orc_tables {
+    uint32_t atmosphere_points, aerosol_points;
+    const double *atmosphere;      /* [n][3]: altitude_km, air_cm3, ozone_cm3 */
+    const double *aerosol;         /* [n][5]: altitude_km, mass_g_m3 x 4      */
+    const double *aerosol_optics;  /* [n_lambda][4][2]: scattering, absorption per g/m^3 */
+    const double *ozone_cross_section_cm2;   /* [n_lambda] */
+    const double *rayleigh_cross_section_m2; /* [n_lambda] or NULL: 5.8e-31 (550/l)^4 */
+    const double *solar_irradiance_w_m2;     /* [n_lambda] or NULL: Planck */
+} orc_tables;
+
 typedef struct orc_params {
     /* Planet (geometry.rs:12-17; the configurations use a 6460 km top). */
     double ground_radius_km;
@@ -91,6 +105,9 @@ typedef struct orc_params {
      * the same delta-tracking law; thin shells near the ground, where the
      * density falls fastest, give tighter majorants for the same count. */
     double shell_growth;
+    /* Tabulated medium (NULL: analytic).  Aerosol phase stays HG (aerosol_g):
+     * the reference's Mie table is absent from its checkout. */
+    const orc_tables *tables;
 } orc_params;
 
 /* Per-wavelength constants derived from the parameters. */
@@ -121,6 +138,9 @@ void orc_sun_direction(const orc_params *p, double dir[3]);
 int orc_majorant_grid(const orc_params *p, double *out /* n_lambda*layers */);
 void orc_coefficients(const orc_params *p, const orc_lambda *lam, double altitude_km,
                       double *sigma_r, double *sigma_as, double *sigma_o, double *sigma_aa);
+/* Same for band index `band`, either medium (tabulated: medium.rs:28-57). */
+void orc_coefficients_band(const orc_params *p, uint32_t band, double altitude_km, double *sigma_r,
+                           double *sigma_as, double *sigma_o, double *sigma_aa);
 
 /* Geometry. */
 int orc_next_boundary_reference(const orc_params *p, const double o[3], const double d[3],
@@ -129,6 +149,9 @@ int orc_segments_reference(const orc_params *p, const double o[3], const double 
                            double *ends, int *layers, int cap);
 int orc_segments_walk(const orc_params *p, const double o[3], const double d[3], double t_max,
                       double *ends, int *layers, int cap);
+/* Same for band index `band` of either medium. */
+double orc_optical_depth_band(const orc_params *p, uint32_t band, const double o[3], const double d[3],
+                              double t_max);
 double orc_optical_depth(const orc_params *p, const orc_lambda *lam, const double o[3],
                          const double d[3], double t_max);
 

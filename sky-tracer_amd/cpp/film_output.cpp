@@ -157,7 +157,9 @@ SpectralRgb spectral_rgb_solar_d65(const SceneData &scene)
     const uint32_t nb = (uint32_t)scene.wavelengths_nm.size();
     sky_wavelengths wl{nb, scene.wavelengths_nm.data(), scene.weights_nm.empty() ? nullptr : scene.weights_nm.data()};
     std::vector<float> rows(8 * (size_t)nb);
-    if (sky_wavelength_table(&scene.atmosphere, &wl, rows.data(), (uint32_t)rows.size()) != SKY_OK)
+    sky_medium_tables tables{};
+    const sky_atmosphere atm = scene.atmosphere_c(tables);
+    if (sky_wavelength_table(&atm, &wl, rows.data(), (uint32_t)rows.size()) != SKY_OK)
         throw RenderError(RenderError::Kind::InvalidConfig, sky_last_error());
     SpectralRgb c;
     c.band_xyz.resize(3 * (size_t)nb);

@@ -9,6 +9,7 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -38,12 +39,40 @@ struct RenderConfig {  // config.rs:2-12, defaults config.rs:38-52
     double shell_growth = 1.0;  // majorant shell thickness growth (1: uniform)
 };
 
-struct SceneData {  // analytic counterpart of sky_core::atmosphere::SceneData
+// The reference's tabulated medium (data.rs:53-152) in the layout of
+// sky_medium_tables.
+struct MediumData {
+    std::vector<double> atmosphere;      // [n][3]: altitude_km, air_cm3, ozone_cm3 (ascending)
+    std::vector<double> aerosol;         // [n][5]: altitude_km, mass_g_m3 of inso, waso, soot, suso
+    std::vector<double> aerosol_optics;  // [bands][4][2]: scattering, absorption per g/m^3
+    std::vector<double> ozone_cross_section_cm2, solar_irradiance_w_m2;  // [bands]
+};
+
+struct SceneData {  // sky_core::atmosphere::SceneData
     sky_planet planet{};
     sky_atmosphere atmosphere{};
     float sun_elevation_deg = 0.0f, sun_azimuth_deg = 0.0f;
     std::vector<double> wavelengths_nm;
     std::vector<double> weights_nm;  // empty: uniform spacing
+    std::shared_ptr<const MediumData> medium;  // null: the analytic profiles
+    // The atmosphere with the tables attached; `storage` must outlive its use.
+    sky_atmosphere atmosphere_c(sky_medium_tables &storage) const
+    {
+        sky_atmosphere a = atmosphere;
+        a.tables = nullptr;
+        if (medium) {
+            storage.atmosphere_points = (uint32_t)(medium->atmosphere.size() / 3);
+            storage.aerosol_points = (uint32_t)(medium->aerosol.size() / 5);
+            storage.atmosphere = medium->atmosphere.data();
+            storage.aerosol = medium->aerosol.data();
+            storage.aerosol_optics = medium->aerosol_optics.data();
+            storage.ozone_cross_section_cm2 = medium->ozone_cross_section_cm2.data();
+            storage.rayleigh_cross_section_m2 = nullptr;
+            storage.solar_irradiance_w_m2 = medium->solar_irradiance_w_m2.data();
+            a.tables = &storage;
+        }
+        return a;
+    }
 };
 
 class Film {  // film.rs:2-62
@@ -73,6 +102,15 @@ public:
 private:
     Kind kind_;
 };
+
+// load_scene_data (data.rs:15-51) from the reference's data directory:
+// bands.csv, atmosphere_profile.csv, aerosol_profile.csv and
+// aerosol_optics.csv; planet 6360 / 6480 km, ground albedo 0.18, sun radius
+// 0.00465047 rad (atmosphere.rs:9-10, geometry.rs:15).  The aerosol phase is
+// Henyey-Greenstein (g 0.8): the reference's mie_phase.csv is not in its
+// checkout.  Throws std::runtime_error naming the file on a missing or
+// malformed table.
+SceneData load_scene_data(const std::string &data_dir, float sun_elevation_deg, float sun_azimuth_deg);
 
 // load_scene_data for the analytic atmosphere: band_count midpoint bands over
 // 380-780 nm, or with band_count 0 the reference's 41 bands at 380..780 nm

@@ -4,13 +4,15 @@
 // the load / render / output / total timing lines.  The default band set is
 // the reference's 41 bands (380..780 nm every 10 nm).  --bands N (N midpoint
 // bands over 380-780 nm), --max-depth and --fast are extensions (path depth
-// limit, carried optical depth + phase sampling).
+// limit, carried optical depth + phase sampling).  --data-dir (default data)
+// loads the reference's scene tables when present (the tabulated medium).
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <sys/stat.h>
 
 #include "film_output.hpp"
 #include "sky.hpp"
@@ -43,7 +45,8 @@ void usage()
         "           [--sun-elevation-deg F] [--sun-azimuth-deg F] [--observer-altitude-km F]\n"
         "           [--direct-light-samples N] [--png-exposure F] [--sky-view-lut] [--rebuild-rgb-only]\n"
         "           [--bands N] [--max-depth N] [--fast]\n"
-        "--data-dir is accepted for flag compatibility: the analytic atmosphere reads no data files.");
+        "--data-dir DIR (default data) holds the reference's tables (bands.csv, atmosphere_profile.csv,\n"
+        "aerosol_profile.csv, aerosol_optics.csv); without them the analytic atmosphere is rendered.");
 }
 
 }  // namespace
@@ -57,7 +60,8 @@ int main(int argc, char **argv)
     bool width_set = false, height_set = false;
     uint32_t bands = 0;  // 0: the reference's 41-band grid
     float png_exposure = 0.05f;
-    std::string out = "out";
+    std::string out = "out", data_dir = "data";
+    bool data_dir_set = false;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
         auto next = [&]() -> const char * {
@@ -72,7 +76,7 @@ int main(int argc, char **argv)
         else if (a == "--spp") cfg.spp = (uint32_t)std::strtoul(next(), nullptr, 10);
         else if (a == "--seed") cfg.seed = std::strtoull(next(), nullptr, 0);
         else if (a == "--out") out = next();
-        else if (a == "--data-dir") next();
+        else if (a == "--data-dir") data_dir = next(), data_dir_set = true;
         else if (a == "--sun-elevation-deg") cfg.sun_elevation_deg = std::strtof(next(), nullptr);
         else if (a == "--sun-azimuth-deg") cfg.sun_azimuth_deg = std::strtof(next(), nullptr);
         else if (a == "--observer-altitude-km") cfg.observer_altitude_km = std::strtof(next(), nullptr);
@@ -95,7 +99,12 @@ int main(int argc, char **argv)
 
     try {
         auto load0 = std::chrono::steady_clock::now();
-        sky::SceneData scene = sky::load_scene_data(cfg.sun_elevation_deg, cfg.sun_azimuth_deg, bands);
+        // The reference's tables when the data directory holds them (an explicit
+        // --data-dir must); otherwise the analytic atmosphere.
+        struct stat st_data;
+        const bool tabulated = data_dir_set || ::stat((data_dir + "/bands.csv").c_str(), &st_data) == 0;
+        sky::SceneData scene = tabulated ? sky::load_scene_data(data_dir, cfg.sun_elevation_deg, cfg.sun_azimuth_deg)
+                                         : sky::load_scene_data(cfg.sun_elevation_deg, cfg.sun_azimuth_deg, bands);
         double load_s = since(load0);
 
         if (rebuild) {  // main.rs:85-104
@@ -112,6 +121,7 @@ int main(int argc, char **argv)
 
         std::printf("config: gpu-integrator output=%s direct-light-samples=%u\n", sky::label(cfg.output_projection),
                     cfg.direct_light_samples);
+        std::printf("medium: %s\n", tabulated ? ("tables from " + data_dir).c_str() : "analytic");
         std::fflush(stdout);
         auto render0 = std::chrono::steady_clock::now();
         sky::Film film = sky::render(scene, cfg);

@@ -58,8 +58,9 @@ struct sky_context {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     sky_tuning tuning{};
     DevBuf lam, maj, shells, film, film_m2, xyz, accum, meta, diag, rays, tau, lamd, jobs, adj_units, adj_dl,
-        adj_grad;
+        adj_grad, tab, b_tab;
     std::vector<sky::DevJob> host_jobs;  // staging, alive until the stream syncs
+    std::vector<uint8_t> host_tab, host_btab;  // medium tables, alive until the stream syncs
     // ragged batches
     DevBuf b_lam, b_maj, b_shells, b_xyz, b_film, b_accum, b_meta, b_jobs;
     hipGraph_t graph = nullptr;
@@ -102,6 +103,13 @@ sky_status run_render(sky_context *ctx, sky::HostScene &hs, hipStream_t stream,
                            hipMemcpyHostToDevice, stream),
             "upload shell table", SKY_ERR_COPY);
     HIP_TRY(sky::launch_zero((uint32_t *)ctx->diag.p, 16, stream), "clear diagnostics", SKY_ERR_SYNC);
+    if (!hs.tab.empty()) {
+        hs.tab.pack(ctx->host_tab);
+        HIP_TRY(ctx->tab.ensure(ctx->host_tab.size()), "alloc medium table", SKY_ERR_OOM);
+        HIP_TRY(hipMemcpyAsync(ctx->tab.p, ctx->host_tab.data(), ctx->host_tab.size(), hipMemcpyHostToDevice, stream),
+                "upload medium table", SKY_ERR_COPY);
+    }
+    hs.tab.attach(s, ctx->tab.p);
     s.lam = (const sky::DevLambda *)ctx->lam.p;
     s.maj = (const float *)ctx->maj.p;
     s.shell_d = (const double *)ctx->shells.p;
@@ -282,6 +290,8 @@ sky_status sky_wavelength_table(const sky_atmosphere *atm, const sky_wavelengths
     if (!atm || !wl || !out || !wl->center_nm || wl->count == 0 || wl->count > 64)
         return fail(SKY_ERR_INVALID_CONFIG, "invalid wavelength table request");
     if (n_out < 8 * wl->count) return fail(SKY_ERR_INVALID_CONFIG, "output too small");
+    std::string err;
+    if (atm->tables && !sky::validate_tables(atm, wl, err)) return fail(SKY_ERR_INVALID_CONFIG, err);
     std::vector<sky::DevLambda> rows;
     sky::fill_wavelength_rows(atm, wl, rows);
     std::memcpy(out, rows.data(), rows.size() * sizeof(sky::DevLambda));
@@ -297,6 +307,8 @@ sky_status sky_majorant_grid(const sky_planet *planet, const sky_atmosphere *atm
         layers == 0 || layers > 256 || !(g >= 1.0 && g <= 16.0) || layers * std::log2(g) > 1000.0)
         return fail(SKY_ERR_INVALID_CONFIG, "invalid majorant grid request");
     if (n_out < layers * wl->count) return fail(SKY_ERR_INVALID_CONFIG, "output too small");
+    std::string err;
+    if (atm->tables && !sky::validate_tables(atm, wl, err)) return fail(SKY_ERR_INVALID_CONFIG, err);
     std::vector<float> maj;
     sky::fill_majorants(planet, atm, wl, layers, margin, g, maj);
     std::memcpy(out, maj.data(), maj.size() * sizeof(float));
@@ -475,6 +487,7 @@ sky_status sky_render_adjoint(sky_context *ctx, const sky_planet *planet, const 
     auto t0 = std::chrono::steady_clock::now();
     sky_status st = sky_validate_config(planet, atm, wl, cfg);
     if (st != SKY_OK) return st;
+    if (atm->tables) return fail(SKY_ERR_INVALID_CONFIG, "the adjoint differentiates the analytic medium only");
     if (!dl_dxyz || !grad) return fail(SKY_ERR_INVALID_CONFIG, "null cotangent or gradient buffer");
     if (!ctx) return fail(SKY_ERR_NO_DEVICE, "no device context");
     HIP_TRY(hipSetDevice(ctx->device), "set device", SKY_ERR_DEVICE_INIT);
@@ -495,6 +508,7 @@ sky_status sky_render_adjoint_device(sky_context *ctx, const sky_planet *planet,
     auto t0 = std::chrono::steady_clock::now();
     sky_status st = sky_validate_config(planet, atm, wl, cfg);
     if (st != SKY_OK) return st;
+    if (atm->tables) return fail(SKY_ERR_INVALID_CONFIG, "the adjoint differentiates the analytic medium only");
     if (!d_dl_dxyz || !grad) return fail(SKY_ERR_INVALID_CONFIG, "null cotangent or gradient buffer");
     if (!ctx) return fail(SKY_ERR_NO_DEVICE, "no device context");
     HIP_TRY(hipSetDevice(ctx->device), "set device", SKY_ERR_DEVICE_INIT);
@@ -533,9 +547,9 @@ sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint3
     std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost(a) > cost(b); });
 
     // Table offsets.
-    size_t n_lam = 0, n_maj = 0, n_shl = 0, n_xyz = 0, n_film = 0, n_acc = 0;
+    size_t n_lam = 0, n_maj = 0, n_shl = 0, n_xyz = 0, n_film = 0, n_acc = 0, n_tab = 0;
     uint32_t max_npx = 0, total_items = 0, max_maj = 0, max_layers = 0;
-    std::vector<size_t> o_lam(n), o_maj(n), o_shl(n), o_xyz(n), o_film(n), o_acc(n);
+    std::vector<size_t> o_lam(n), o_maj(n), o_shl(n), o_xyz(n), o_film(n), o_acc(n), o_tab(n);
     ctx->host_jobs.assign(n, sky::DevJob{});
     const uint32_t min_items = 0;  // the batch balances across renders; no sample split needed
     uint64_t samples = 0;
@@ -548,6 +562,8 @@ sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint3
         o_shl[i] = n_shl;
         o_xyz[i] = n_xyz;
         o_film[i] = n_film;
+        o_tab[i] = n_tab;
+        if (!h.tab.empty()) n_tab += h.tab.bytes();
         n_lam += h.lam.size();
         n_maj += h.maj.size();
         n_shl += h.shell_d.size();
@@ -574,6 +590,8 @@ sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint3
     HIP_TRY(ctx->b_accum.ensure(n_acc * sizeof(float2)), "alloc batch partial sums", SKY_ERR_OOM);
     HIP_TRY(ctx->b_meta.ensure(meta_words * sizeof(uint32_t)), "alloc batch counters", SKY_ERR_OOM);
     HIP_TRY(ctx->b_jobs.ensure(n * sizeof(sky::DevJob)), "alloc batch job table", SKY_ERR_OOM);
+    if (n_tab) HIP_TRY(ctx->b_tab.ensure(n_tab), "alloc batch medium tables", SKY_ERR_OOM);
+    ctx->host_btab.assign(n_tab, 0);
     HIP_TRY(ctx->diag.ensure(64), "alloc diagnostics", SKY_ERR_OOM);
     std::vector<sky::DevLambda> all_lam(n_lam);
     std::vector<float> all_maj(n_maj);
@@ -589,6 +607,12 @@ sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint3
         j.s.lam = (const sky::DevLambda *)ctx->b_lam.p + o_lam[i];
         j.s.maj = (const float *)ctx->b_maj.p + o_maj[i];
         j.s.shell_d = (const double *)ctx->b_shells.p + o_shl[i];
+        if (!h.tab.empty()) {
+            std::vector<uint8_t> blob;
+            h.tab.pack(blob);
+            std::copy(blob.begin(), blob.end(), ctx->host_btab.begin() + o_tab[i]);
+        }
+        h.tab.attach(j.s, (const uint8_t *)ctx->b_tab.p + o_tab[i]);
         j.s.xyz = (float *)ctx->b_xyz.p + o_xyz[i];
         j.s.film = items[i].film ? (float *)ctx->b_film.p + o_film[i] : nullptr;
         j.s.film_m2 = nullptr;
@@ -605,6 +629,9 @@ sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint3
             "upload batch shell tables", SKY_ERR_COPY);
     HIP_TRY(hipMemcpyAsync(ctx->b_jobs.p, ctx->host_jobs.data(), n * sizeof(sky::DevJob), hipMemcpyHostToDevice,
                            stream), "upload batch job table", SKY_ERR_COPY);
+    if (n_tab)
+        HIP_TRY(hipMemcpyAsync(ctx->b_tab.p, ctx->host_btab.data(), n_tab, hipMemcpyHostToDevice, stream),
+                "upload batch medium tables", SKY_ERR_COPY);
     HIP_TRY(sky::launch_zero((uint32_t *)ctx->diag.p, 16, stream), "clear diagnostics", SKY_ERR_SYNC);
     sky::WavePlan plan = sky::plan_launch(ctx->tuning, ctx->num_cus, total_items, max_maj, max_layers);
 
@@ -764,6 +791,14 @@ sky_status sky_optical_depth(sky_context *ctx, const sky_planet *planet, const s
     HIP_TRY(hipMemcpyAsync(ctx->lamd.p, lamd.data(), lamd.size() * sizeof(double), hipMemcpyHostToDevice,
                            ctx->stream),
             "upload coefficients", SKY_ERR_COPY);
+    if (!hs.tab.empty()) {
+        hs.tab.pack(ctx->host_tab);
+        HIP_TRY(ctx->tab.ensure(ctx->host_tab.size()), "alloc medium table", SKY_ERR_OOM);
+        HIP_TRY(hipMemcpyAsync(ctx->tab.p, ctx->host_tab.data(), ctx->host_tab.size(), hipMemcpyHostToDevice,
+                               ctx->stream),
+                "upload medium table", SKY_ERR_COPY);
+    }
+    hs.tab.attach(hs.dev, ctx->tab.p);
     HIP_TRY(sky::launch_optical_depth(hs.dev, (const double *)ctx->rays.p, n, (const double *)ctx->lamd.p,
                                       (double *)ctx->tau.p, ctx->stream),
             "launch optical depth", SKY_ERR_SYNC);

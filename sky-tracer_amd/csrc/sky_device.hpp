@@ -59,6 +59,15 @@ struct DevScene {
     uint32_t seed_lo, seed_hi;
     uint32_t nlam, L;
     uint32_t dls, projection, max_depth, rr_start, watchdog_limit, max_segments, flags;
+    // Tabulated medium (null tab: the analytic profiles): sigma (R, As, O,
+    // Aa) per band at K altitude breakpoints, linear in between; a uniform
+    // index of tab_nidx cells gives each altitude's first candidate interval.
+    const float4 *tab;         // [lambda * K + k]
+    const float *tab_z;        // [K] breakpoints (km), tab_z[0] = 0, tab_z[K-1] = top
+    const float *tab_inv;      // [K] 1 / (tab_z[k+1] - tab_z[k])
+    const uint16_t *tab_idx;   // [tab_nidx]
+    uint32_t tab_k, tab_nidx;
+    float tab_inv_dh;
     const DevLambda *lam;
     const float *maj;  // [lambda * L + layer]
     const double *shell_d;  // [L + 1]: r_k^2 - R^2 = h_k (2R + h_k)
@@ -353,10 +362,35 @@ struct Coeffs {
     float sr, sas, so, saa;
 };
 
-__device__ __forceinline__ Coeffs medium4(const DevScene &s, float br, float bo, float bas, float baa, float h)
+// Interval k of the breakpoint grid holding altitude h (0 <= h <= top).
+__device__ __forceinline__ uint32_t tab_interval(const DevScene &s, float h)
+{
+    uint32_t j = (uint32_t)(h * s.tab_inv_dh);
+    j = j < s.tab_nidx ? j : s.tab_nidx - 1;
+    uint32_t k = s.tab_idx[j];
+    while (k + 2 < s.tab_k && h >= s.tab_z[k + 1]) ++k;
+    while (k > 0 && h < s.tab_z[k]) --k;
+    return k;
+}
+
+// coefficients_at (medium.rs:28-57) for the tabulated profiles: the merged
+// breakpoint grid makes every coefficient linear within an interval.
+__device__ __forceinline__ Coeffs tab_medium(const DevScene &s, uint32_t band, float h)
+{
+    const uint32_t k = tab_interval(s, h);
+    float t = (h - s.tab_z[k]) * s.tab_inv[k];
+    t = fminf(fmaxf(t, 0.0f), 1.0f);
+    const float4 a = s.tab[band * s.tab_k + k], b = s.tab[band * s.tab_k + k + 1];
+    return Coeffs{fmaf(b.x - a.x, t, a.x), fmaf(b.y - a.y, t, a.y), fmaf(b.z - a.z, t, a.z),
+                  fmaf(b.w - a.w, t, a.w)};
+}
+
+__device__ __forceinline__ Coeffs medium4(const DevScene &s, uint32_t band, float br, float bo, float bas, float baa,
+                                          float h)
 {
     Coeffs c{0.0f, 0.0f, 0.0f, 0.0f};
     if (h < 0.0f || h > s.topf) return c;
+    if (s.tab) return tab_medium(s, band, h);
     float rr = f_exp2(-h * s.l2_hr);
     float ra = f_exp2(-h * s.l2_ha);
     float ro = 1.0f - fabsf(h - s.hc) * s.inv_hw;
@@ -368,9 +402,9 @@ __device__ __forceinline__ Coeffs medium4(const DevScene &s, float br, float bo,
     return c;
 }
 
-__device__ __forceinline__ Coeffs medium(const DevScene &s, const DevLambda &l, float h)
+__device__ __forceinline__ Coeffs medium(const DevScene &s, uint32_t band, const DevLambda &l, float h)
 {
-    return medium4(s, l.br, l.bo, l.bas, l.baa, h);
+    return medium4(s, band, l.br, l.bo, l.bas, l.baa, h);
 }
 
 // ------------------------------------------------------------ phase

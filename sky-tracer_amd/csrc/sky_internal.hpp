@@ -14,11 +14,35 @@ namespace sky {
 
 // Host copy of one render's packed scene (integrator.rs:319-418 PackedScene):
 // the device struct plus the per-wavelength and majorant tables it points to.
+// Tabulated medium on the merged breakpoint grid of both profiles (every
+// coefficient is linear between consecutive breakpoints), packed for the
+// device as one blob: sigma [band][K][4] | z [K] | 1/len [K] | index [nidx].
+struct MediumTable {
+    std::vector<float> sig, z, inv;
+    std::vector<uint16_t> idx;
+    float inv_dh = 0.0f;
+    bool empty() const { return z.empty(); }
+    size_t off_z() const { return (sig.size() * 4 + 15) & ~size_t(15); }
+    size_t off_inv() const { return off_z() + ((z.size() * 4 + 15) & ~size_t(15)); }
+    size_t off_idx() const { return off_inv() + ((inv.size() * 4 + 15) & ~size_t(15)); }
+    size_t bytes() const { return off_idx() + ((idx.size() * 2 + 15) & ~size_t(15)); }
+    void pack(std::vector<uint8_t> &blob) const;
+    // Points the scene's tab* fields into a device copy of the blob.
+    void attach(DevScene &s, const void *dev) const;
+};
+
+// The reference's coefficients_at (medium.rs:28-57) in FP64: sigma R, As,
+// O, Aa (km^-1) of band b at altitude h from the raw profiles.
+void table_coeffs(const sky_medium_tables *t, uint32_t band, double lambda_nm, double h, double out[4]);
+// Checks the tables of atm (when present) against the wavelength set.
+bool validate_tables(const sky_atmosphere *atm, const sky_wavelengths *wl, std::string &err);
+
 struct HostScene {
     DevScene dev{};
     std::vector<DevLambda> lam;
     std::vector<float> maj;
     std::vector<double> shell_d;  // [L + 1] r_k^2 - R^2 of the shell boundaries
+    MediumTable tab;              // empty: analytic medium
     uint32_t nlam = 0;
 };
 

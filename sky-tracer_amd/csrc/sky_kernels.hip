@@ -71,13 +71,24 @@ struct OdLambda {
     double br, bo, bas, baa;
 };
 
-__device__ double od_sigma(const DevScene &s, const OdLambda &l, double hr, double ha, double hc,
+__device__ double od_sigma(const DevScene &s, uint32_t band, const OdLambda &l, double hr, double ha, double hc,
                            double hw, double b, double q, double t)
 {
     double sv = b + t;
     double w = q + sv * sv;
     double h = w / (sqrt(s.R * s.R + w) + s.R);
     if (h < 0.0 || h > s.top) return 0.0;
+    if (s.tab) {  // tabulated medium: the same breakpoint table as the tracer, lerped in FP64
+        uint32_t k = tab_interval(s, (float)h);
+        while (k + 2 < s.tab_k && h >= (double)s.tab_z[k + 1]) ++k;
+        while (k > 0 && h < (double)s.tab_z[k]) --k;
+        const double z0 = s.tab_z[k], z1 = s.tab_z[k + 1];
+        double t = (h - z0) / (z1 - z0);
+        t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+        const float4 a = s.tab[band * s.tab_k + k], c = s.tab[band * s.tab_k + k + 1];
+        return (double)a.x + ((double)c.x - a.x) * t + (double)a.y + ((double)c.y - a.y) * t + (double)a.z +
+               ((double)c.z - a.z) * t + (double)a.w + ((double)c.w - a.w) * t;
+    }
     double ro = 1.0 - fabs(h - hc) / hw;
     ro = ro < 0.0 ? 0.0 : ro;
     return l.br * exp(-h / hr) + (l.bas + l.baa) * exp(-h / ha) + l.bo * ro;
@@ -113,8 +124,9 @@ __global__ void k_optical_depth(DevScene s, const double *rays, uint32_t n, cons
     double t = 0.0;
     while (t < t_max) {
         double next = t_max;
-        for (int k = 0; k <= (int)s.L + 3; ++k) {
-            double hk = k <= (int)s.L ? k * s.dz : kinks[k - (int)s.L - 1];
+        const int extra = s.tab ? (int)s.tab_k : 3;  // the table's breakpoints, or the ozone tent's kinks
+        for (int k = 0; k <= (int)s.L + extra; ++k) {
+            double hk = k <= (int)s.L ? k * s.dz : (s.tab ? (double)s.tab_z[k - (int)s.L - 1] : kinks[k - (int)s.L - 1]);
             double disc = hk * (2.0 * s.R + hk) - q;
             if (disc <= 0.0) continue;
             double sq = sqrt(disc);
@@ -129,8 +141,8 @@ __global__ void k_optical_depth(DevScene s, const double *rays, uint32_t n, cons
             double a = t + (next - t) * j / m, e = t + (next - t) * (j + 1) / m;
             double c = 0.5 * (a + e), hwid = 0.5 * (e - a), acc = 0.0;
             for (int i = 0; i < 8; ++i)
-                acc += gw[i] * (od_sigma(s, l, hr, ha, hc, hw, b, q, c - hwid * gx[i]) +
-                                od_sigma(s, l, hr, ha, hc, hw, b, q, c + hwid * gx[i]));
+                acc += gw[i] * (od_sigma(s, band, l, hr, ha, hc, hw, b, q, c - hwid * gx[i]) +
+                                od_sigma(s, band, l, hr, ha, hc, hw, b, q, c + hwid * gx[i]));
             total += acc * hwid;
         }
         t = next;

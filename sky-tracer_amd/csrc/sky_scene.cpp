@@ -5,6 +5,7 @@
 // the analytic atmosphere: per-wavelength coefficients, the Planck sun, the
 // CIE weights, the per-shell majorant grid (medium.rs:63-83) and the constant
 // block the kernels read.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -65,6 +66,144 @@ static void coeffs_d(const sky_atmosphere *a, double l, double c[4])
     c[3] = a->aerosol_abs_550_m_inv * 1000.0 * ang;
 }
 
+// bracket + lerp of one profile column at altitude x (medium.rs:108-155):
+// clamped to the end points, t = (x - z_lo) / (z_hi - z_lo).
+static double profile_at(const double *rows, uint32_t n, uint32_t stride, uint32_t col, double x)
+{
+    uint32_t hi = 0;
+    while (hi < n && rows[(size_t)hi * stride] < x) ++hi;  // partition_point(z < x)
+    if (hi == 0) return rows[col];
+    if (hi >= n) return rows[(size_t)(n - 1) * stride + col];
+    const double *lo = rows + (size_t)(hi - 1) * stride, *up = rows + (size_t)hi * stride;
+    double t = up[0] > lo[0] ? (x - lo[0]) / (up[0] - lo[0]) : 0.0;
+    t = std::fmin(std::fmax(t, 0.0), 1.0);
+    return lo[col] * (1.0 - t) + up[col] * t;
+}
+
+void table_coeffs(const sky_medium_tables *t, uint32_t b, double lambda_nm, double h, double out[4])
+{
+    const double air = profile_at(t->atmosphere, t->atmosphere_points, 3, 1, h);
+    const double o3 = profile_at(t->atmosphere, t->atmosphere_points, 3, 2, h);
+    const double r = 550.0 / lambda_nm;
+    const double xs_r = t->rayleigh_cross_section_m2 ? t->rayleigh_cross_section_m2[b] : 5.8e-31 * r * r * r * r;
+    out[0] = std::fmax(air * xs_r * 1.0e9, 0.0);
+    out[2] = std::fmax(o3 * t->ozone_cross_section_cm2[b] * 1.0e5, 0.0);
+    out[1] = out[3] = 0.0;
+    for (int sp = 0; sp < SKY_AEROSOL_SPECIES; ++sp) {
+        const double m = std::fmax(profile_at(t->aerosol, t->aerosol_points, 5, 1 + sp, h), 0.0);
+        out[1] += m * t->aerosol_optics[((size_t)b * SKY_AEROSOL_SPECIES + sp) * 2];
+        out[3] += m * t->aerosol_optics[((size_t)b * SKY_AEROSOL_SPECIES + sp) * 2 + 1];
+    }
+}
+
+// Merged breakpoint grid: 0, top and every profile altitude in between.
+static std::vector<double> table_breakpoints(const sky_medium_tables *t, double top)
+{
+    std::vector<double> z{0.0, top};
+    for (uint32_t i = 0; i < t->atmosphere_points; ++i) z.push_back(t->atmosphere[3 * (size_t)i]);
+    for (uint32_t i = 0; i < t->aerosol_points; ++i) z.push_back(t->aerosol[5 * (size_t)i]);
+    std::sort(z.begin(), z.end());
+    std::vector<double> out;
+    for (double v : z)
+        if (v >= 0.0 && v <= top && (out.empty() || v > out.back() + 1e-9)) out.push_back(v);
+    if (out.back() < top) out.push_back(top);
+    return out;
+}
+
+static void build_medium_table(const sky_atmosphere *a, const sky_wavelengths *wl, double top, MediumTable &m)
+{
+    const sky_medium_tables *t = a->tables;
+    const std::vector<double> z = table_breakpoints(t, top);
+    const size_t K = z.size();
+    m.z.resize(K);
+    m.inv.assign(K, 0.0f);
+    for (size_t k = 0; k < K; ++k) {
+        m.z[k] = (float)z[k];
+        if (k + 1 < K) m.inv[k] = (float)(1.0 / (z[k + 1] - z[k]));
+    }
+    m.sig.resize((size_t)wl->count * K * 4);
+    for (uint32_t b = 0; b < wl->count; ++b)
+        for (size_t k = 0; k < K; ++k) {
+            double c[4];
+            table_coeffs(t, b, wl->center_nm[b], z[k], c);
+            for (int i = 0; i < 4; ++i) m.sig[((size_t)b * K + k) * 4 + i] = (float)c[i];
+        }
+    const uint32_t nidx = 1024;
+    m.idx.resize(nidx);
+    m.inv_dh = (float)(nidx / top);
+    size_t k = 0;
+    for (uint32_t j = 0; j < nidx; ++j) {
+        const double x = j * (top / nidx);
+        while (k + 2 < K && z[k + 1] <= x) ++k;
+        m.idx[j] = (uint16_t)k;
+    }
+}
+
+void MediumTable::pack(std::vector<uint8_t> &blob) const
+{
+    blob.assign(bytes(), 0);
+    std::memcpy(blob.data(), sig.data(), sig.size() * 4);
+    std::memcpy(blob.data() + off_z(), z.data(), z.size() * 4);
+    std::memcpy(blob.data() + off_inv(), inv.data(), inv.size() * 4);
+    std::memcpy(blob.data() + off_idx(), idx.data(), idx.size() * 2);
+}
+
+void MediumTable::attach(DevScene &s, const void *dev) const
+{
+    if (empty()) {
+        s.tab = nullptr;
+        return;
+    }
+    const uint8_t *p = (const uint8_t *)dev;
+    s.tab = (const float4 *)p;
+    s.tab_z = (const float *)(p + off_z());
+    s.tab_inv = (const float *)(p + off_inv());
+    s.tab_idx = (const uint16_t *)(p + off_idx());
+    s.tab_k = (uint32_t)z.size();
+    s.tab_nidx = (uint32_t)idx.size();
+    s.tab_inv_dh = inv_dh;
+}
+
+bool validate_tables(const sky_atmosphere *a, const sky_wavelengths *wl, std::string &err)
+{
+    const sky_medium_tables *t = a->tables;
+    if (!t->atmosphere || !t->aerosol || !t->aerosol_optics || !t->ozone_cross_section_cm2 ||
+        t->atmosphere_points == 0 || t->aerosol_points == 0 || t->atmosphere_points > 4096 ||
+        t->aerosol_points > 4096) {
+        err = "medium tables need 1..4096 profile points, optics and ozone cross sections";
+        return false;
+    }
+    auto check_profile = [&](const double *rows, uint32_t n, uint32_t stride) {
+        for (uint32_t i = 0; i < n; ++i)
+            for (uint32_t c = 0; c < stride; ++c) {
+                const double v = rows[(size_t)i * stride + c];
+                if (!std::isfinite(v) || (c > 0 && v < 0.0)) return false;
+                if (c == 0 && i > 0 && v < rows[(size_t)(i - 1) * stride]) return false;
+            }
+        return true;
+    };
+    if (!check_profile(t->atmosphere, t->atmosphere_points, 3) || !check_profile(t->aerosol, t->aerosol_points, 5)) {
+        err = "profiles must be finite, non-negative and in ascending altitude";
+        return false;
+    }
+    for (uint32_t b = 0; b < wl->count; ++b) {
+        bool ok = std::isfinite(t->ozone_cross_section_cm2[b]) && t->ozone_cross_section_cm2[b] >= 0.0;
+        if (t->rayleigh_cross_section_m2)
+            ok = ok && std::isfinite(t->rayleigh_cross_section_m2[b]) && t->rayleigh_cross_section_m2[b] >= 0.0;
+        if (t->solar_irradiance_w_m2)
+            ok = ok && std::isfinite(t->solar_irradiance_w_m2[b]) && t->solar_irradiance_w_m2[b] >= 0.0;
+        for (int i = 0; i < 2 * SKY_AEROSOL_SPECIES; ++i) {
+            const double v = t->aerosol_optics[(size_t)b * 2 * SKY_AEROSOL_SPECIES + i];
+            ok = ok && std::isfinite(v) && v >= 0.0;
+        }
+        if (!ok) {
+            err = "cross sections, optics and irradiance must be finite and non-negative";
+            return false;
+        }
+    }
+    return true;
+}
+
 void fill_wavelength_rows(const sky_atmosphere *a, const sky_wavelengths *wl,
                           std::vector<DevLambda> &rows)
 {
@@ -75,13 +214,28 @@ void fill_wavelength_rows(const sky_atmosphere *a, const sky_wavelengths *wl,
         double w = wl->weight_nm ? wl->weight_nm[i] : dw;
         double cmf[3], c[4];
         cmf_xyz(l, cmf);
-        coeffs_d(a, l, c);
         DevLambda &d = rows[i];
+        if (a->tables) {  // sea-level coefficients, for inspection; the kernels read the table
+            double t4[4];
+            table_coeffs(a->tables, i, l, 0.0, t4);
+            c[0] = t4[0];
+            c[1] = t4[2];
+            c[2] = t4[1];
+            c[3] = t4[3];
+        } else {
+            coeffs_d(a, l, c);
+        }
         d.br = (float)c[0];
         d.bo = (float)c[1];
         d.bas = (float)c[2];
         d.baa = (float)c[3];
-        d.lsun = (float)planck(l, a->sun_temperature_k);
+        if (a->tables && a->tables->solar_irradiance_w_m2) {
+            // band irradiance over the sun's solid angle 2 pi (1 - cos r) (data.rs:30-33)
+            const double hs = std::sin(0.5 * a->sun_angular_radius_rad);
+            d.lsun = (float)(a->tables->solar_irradiance_w_m2[i] / (4.0 * kPiD * hs * hs));
+        } else {
+            d.lsun = (float)planck(l, a->sun_temperature_k);
+        }
         d.wx = (float)(cmf[0] * w);
         d.wy = (float)(cmf[1] * w);
         d.wz = (float)(cmf[2] * w);
@@ -111,6 +265,27 @@ void fill_majorants(const sky_planet *planet, const sky_atmosphere *a, const sky
 {
     double top = planet->atmosphere_radius_km - planet->ground_radius_km;
     out.resize((size_t)wl->count * layers);
+    if (a->tables) {
+        // compute_majorant_grid (medium.rs:63-102): the maximum over the shell's
+        // ends, its middle and every profile altitude inside, which is exact
+        // for coefficients linear between profile altitudes.
+        const std::vector<double> z = table_breakpoints(a->tables, top);
+        for (uint32_t b = 0; b < wl->count; ++b)
+            for (uint32_t k = 0; k < layers; ++k) {
+                const double lo = shell_height(top, layers, growth, k), hi = shell_height(top, layers, growth, k + 1);
+                std::vector<double> probes{lo, hi, 0.5 * (lo + hi)};
+                for (double v : z)
+                    if (v > lo && v < hi) probes.push_back(v);
+                double m = 0.0;
+                for (double h : probes) {
+                    double c[4];
+                    table_coeffs(a->tables, b, wl->center_nm[b], h, c);
+                    m = std::fmax(m, c[0] + c[1] + c[2] + c[3]);
+                }
+                out[(size_t)b * layers + k] = (float)std::fmax(m * margin, 1e-8);
+            }
+        return;
+    }
     for (size_t b = 0; b < wl->count; ++b) {
         double c[4];
         coeffs_d(a, wl->center_nm[b], c);
@@ -204,6 +379,13 @@ sky_status build_scene(const sky_planet *planet, const sky_atmosphere *a, const 
     if (!(growth >= 1.0 && growth <= 16.0) || cfg->majorant_layers * std::log2(growth) > 1000.0) {
         err = "shell growth must lie in [1, 16] (0 means 1)";
         return SKY_ERR_INVALID_CONFIG;
+    }
+
+    if (a->tables) {
+        if (!validate_tables(a, wl, err)) return SKY_ERR_INVALID_CONFIG;
+        build_medium_table(a, wl, planet->atmosphere_radius_km - planet->ground_radius_km, out.tab);
+    } else {
+        out.tab = MediumTable{};
     }
 
     out.nlam = wl->count;

@@ -19,7 +19,7 @@ enum TrackMode { kTrackCollision = 0, kTrackDelta = 1, kTrackRatio = 2 };
 // Reference form: a fresh free flight per majorant segment, discarded on
 // overshoot (wgsl:421-436).  kFlagCarriedTau carries the unused majorant
 // optical depth into the next segment instead (same law, fewer draws).
-__device__ __forceinline__ bool track(const DevScene &s, const DevLambda &l, const float *maj_row,
+__device__ __forceinline__ bool track(const DevScene &s, uint32_t band, const DevLambda &l, const float *maj_row,
                                       const Ray &r, Rng &rng, int mode, float &t_hit, Coeffs &co,
                                       float &T, bool &wd)
 {
@@ -56,7 +56,7 @@ __device__ __forceinline__ bool track(const DevScene &s, const DevLambda &l, con
                 t = t + f_div(free_flight(rng_next(rng)), maj);
                 if (t >= seg_end) break;
             }
-            Coeffs c = medium(s, l, altitude(s, r, t));
+            Coeffs c = medium(s, band, l, altitude(s, r, t));
             float st = c.sr + c.sas + c.so + c.saa;
             float ratio = fminf(fmaxf(f_div(st, maj), 0.0f), 1.0f);
             if (mode == kTrackRatio) {
@@ -79,12 +79,12 @@ __device__ __forceinline__ bool track(const DevScene &s, const DevLambda &l, con
     }
 }
 
-__device__ __forceinline__ float transmittance(const DevScene &s, const DevLambda &l,
+__device__ __forceinline__ float transmittance(const DevScene &s, uint32_t band, const DevLambda &l,
                                                const float *maj_row, const Ray &r, Rng &rng, bool &wd)
 {
     float T, th;
     Coeffs co;
-    track(s, l, maj_row, r, rng, (s.flags & kFlagRatio) ? kTrackRatio : kTrackDelta, th, co, T, wd);
+    track(s, band, l, maj_row, r, rng, (s.flags & kFlagRatio) ? kTrackRatio : kTrackDelta, th, co, T, wd);
     return T;
 }
 
@@ -126,7 +126,7 @@ __device__ float trace_path(const DevScene &s, uint32_t x, uint32_t y, uint32_t 
         if (!ray.hit) break;
         float th, Tdummy;
         Coeffs co;
-        if (track(s, l, maj_row, ray, rng, kTrackCollision, th, co, Tdummy, wd)) {
+        if (track(s, band, l, maj_row, ray, rng, kTrackCollision, th, co, Tdummy, wd)) {
             // ray.d is FP32-normalised; positions use its FP64 unit vector.
             double dx = ray.d.x, dy = ray.d.y, dz = ray.d.z;
             double inv = inv_norm64(dx, dy, dz);
@@ -148,7 +148,7 @@ __device__ float trace_path(const DevScene &s, uint32_t x, uint32_t y, uint32_t 
                 ray_setup(s, global_shells(s), px + (double)ld.x * kRayEpsilonKm, py + (double)ld.y * kRayEpsilonKm,
                           pz + (double)ld.z * kRayEpsilonKm, ld, sr);
                 if (sr.hit && sr.kind == 1) {
-                    float T = transmittance(s, l, maj_row, sr, rng, wd);
+                    float T = transmittance(s, band, l, maj_row, sr, rng, wd);
                     float ph = mixed_phase(s, co.sr, co.sas, dot3(ray.d, ld));
                     sum = sum + l.lsun * T * ph / s.cone_pdf;
                 }
@@ -199,7 +199,7 @@ __device__ float trace_path(const DevScene &s, uint32_t x, uint32_t y, uint32_t 
                     ray_setup(s, global_shells(s), px + (double)ld.x * kRayEpsilonKm, py + (double)ld.y * kRayEpsilonKm,
                               pz + (double)ld.z * kRayEpsilonKm, ld, sr);
                     if (sr.hit && sr.kind == 1) {
-                        float T = transmittance(s, l, maj_row, sr, rng, wd);
+                        float T = transmittance(s, band, l, maj_row, sr, rng, wd);
                         sum = sum + s.albedo * kInvPi * l.lsun * T * cs / s.cone_pdf;
                     }
                 }

@@ -250,7 +250,7 @@ __device__ __forceinline__ float lane_altitude(const DevScene &s, const LaneCtx 
 
 __device__ __forceinline__ Coeffs lane_medium(const DevScene &s, const LaneCtx &c, float h)
 {
-    return medium4(s, c.br, c.bo, c.bas, c.baa, h);
+    return medium4(s, c.band, c.br, c.bo, c.bas, c.baa, h);
 }
 
 // One tracking step.  The walk consumes shell crossings (no draw for an

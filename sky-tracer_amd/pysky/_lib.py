@@ -52,6 +52,20 @@ class Atmosphere(C.Structure):
             "sun_temperature_k",
             "sun_angular_radius_rad",
         )
+    ] + [("tables", C.c_void_p)]
+
+
+class MediumTablesC(C.Structure):
+    """sky_medium_tables (include/sky_hip.h)."""
+    _fields_ = [
+        ("atmosphere_points", C.c_uint32),
+        ("aerosol_points", C.c_uint32),
+        ("atmosphere", C.POINTER(C.c_double)),
+        ("aerosol", C.POINTER(C.c_double)),
+        ("aerosol_optics", C.POINTER(C.c_double)),
+        ("ozone_cross_section_cm2", C.POINTER(C.c_double)),
+        ("rayleigh_cross_section_m2", C.POINTER(C.c_double)),
+        ("solar_irradiance_w_m2", C.POINTER(C.c_double)),
     ]
 
 

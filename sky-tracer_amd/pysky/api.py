@@ -158,11 +158,19 @@ class AtmosphereParams:
     ground_albedo: float = 0.3
     sun_temperature_k: float = 5778.0
     sun_angular_radius_rad: float = 0.2675 * np.pi / 180.0
+    # Tabulated medium (pysky.tables.MediumTables, e.g. from
+    # pysky.tables.load_reference_scene); None: the analytic profiles above.
+    tables: Optional[object] = None
 
     def to_c(self) -> L.Atmosphere:
         a = L.Atmosphere()
         for name, _ in L.Atmosphere._fields_:
-            setattr(a, name, float(getattr(self, name)))
+            if name != "tables":
+                setattr(a, name, float(getattr(self, name)))
+        if self.tables is not None:
+            tc = self.tables.to_c()
+            a.tables = C.cast(C.pointer(tc), C.c_void_p)
+            a._keep = tc
         return a
 
 

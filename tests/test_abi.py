@@ -35,7 +35,8 @@ def test_library_exports_every_declared_symbol():
 def test_struct_sizes_match_header_layout():
     # the Python mirrors must have the C layout (no padding surprises)
     assert C.sizeof(L.RenderConfigC) == 4 * 4 + 8 + 3 * 8 + 8 * 4 + 8 + 8 + 8
-    assert C.sizeof(L.Atmosphere) == 15 * 8
+    assert C.sizeof(L.Atmosphere) == 15 * 8 + 8
+    assert C.sizeof(L.MediumTablesC) == 8 + 6 * 8
     assert C.sizeof(L.Tuning) == 5 * 4
 
 
