@@ -121,16 +121,21 @@ def copy_calibration(mib):
 SKYVIEW_ELEVATIONS = (-2.0, -1.0, 0.0, 1.0, 2.0, 5.0, 10.0, 20.0, 45.0, 90.0)
 
 
-def skyview_sweep(spp, mode):
+def skyview_sweep(spp, mode, data_dir=None):
     """The reference's sky-view LUT bake (scripts/bake_skyview_elevations.ps1:2-35):
     10 sun elevations, 256x256 sky-view LUT projection, 41 bands 380..780 nm
-    (10 nm wide), observer 0.2 km, one sun sample, uncapped depth."""
+    (10 nm wide), observer 0.2 km, one sun sample, uncapped depth.  With
+    `data_dir`, the scene is the reference's own tables (tabulated medium)."""
     import pysky
+    from pysky import tables
 
     lam = [380.0 + 10.0 * i for i in range(41)]
     out = []
     for el in SKYVIEW_ELEVATIONS:
-        scene = pysky.SceneData(sun_elevation_deg=el, wavelengths_nm=lam, weights_nm=[10.0] * 41)
+        if data_dir:
+            scene = tables.load_reference_scene(data_dir, el)
+        else:
+            scene = pysky.SceneData(sun_elevation_deg=el, wavelengths_nm=lam, weights_nm=[10.0] * 41)
         cfg = pysky.RenderConfig(width=256, height=256, spp=spp, seed=0x5EC720260430, max_depth=0,
                                  output_projection=pysky.OutputProjection.SkyViewLut, **MODES[mode])
         out.append((scene, cfg))
@@ -149,9 +154,12 @@ def bench_batch(args, ctx):
     from pysky.torch_ops import render_device
 
     if args.workload == "skyview":
-        batch = skyview_sweep(args.spp or 4096, args.mode)
+        data_dir = None if args.medium == "analytic" else args.data_dir
+        batch = skyview_sweep(args.spp or 4096, args.mode, data_dir)
         what = (f"skyview: the reference's LUT bake, 10 sun elevations -2..90 deg, 256x256 sky-view LUT, "
-                f"41 bands 380-780 nm, {args.spp or 4096} spp, uncapped depth, one launch, HIP graph")
+                f"41 bands 380-780 nm, {args.spp or 4096} spp, uncapped depth, one launch, HIP graph, "
+                + ("analytic atmosphere" if data_dir is None else
+                   "the reference's tables (AFGL + OPAC, HG aerosol phase)"))
     else:
         batch = ragged_batch(n=256, seed=1, spp=args.spp or 256, max_depth=32)
         what = ("c4: 256 ragged renders (sun U[-6,90] deg, 64x32..512x256, 3..64 lambda in "
@@ -196,7 +204,8 @@ def bench_batch(args, ctx):
         "unit": "path x lambda samples/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 ray set-up)",
-        "data": "synthetic (analytic atmosphere)",
+        "data": ("the reference's scene tables (data/*.csv), HG aerosol phase"
+                 if args.workload == "skyview" and args.medium == "tables" else "synthetic (analytic atmosphere)"),
         "config": {"workload": what, "estimator": args.mode if args.workload == "skyview" else "carried_is",
                    "majorant_layers": layers, "shell_growth": growth},
         "kernel_ms": float(np.mean(kms)),
@@ -452,6 +461,10 @@ def main():
     ap.add_argument("--spp", type=int, default=0, help="override samples per pixel")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--primaries", type=int, default=6, help="scan: bands per combo (6: all 4,496,388 combos)")
+    ap.add_argument("--medium", default="analytic", choices=["analytic", "tables"],
+                    help="skyview: the analytic atmosphere or the reference's scene tables")
+    ap.add_argument("--data-dir", default=os.path.join(ROOT, "tests", "golden", "reference_data"),
+                    help="directory of the reference's scene tables (bands.csv, profiles, optics)")
     ap.add_argument("--accuracy", action="store_true",
                     help="c2 only: relative RMSE of the last render's XYZ against the 65536-spp FP64 render "
                          "of tests/golden/c2_fp64_reference.npz")
@@ -483,6 +496,13 @@ def main():
                                pysky.RenderConfig(width=1024, height=512, carried_tau=True,
                                                   phase_importance_sampling=True))
             autotune.apply(ctx, pysky.RenderConfig(), p)
+            if args.workload == "skyview" and args.medium == "tables":
+                # shells tuned for the reference's tables (aerosol concentrated near the ground)
+                from pysky import tables
+                pt = autotune.tuned(tables.load_reference_scene(args.data_dir, 5.0),
+                                    pysky.RenderConfig(width=256, height=256, carried_tau=True,
+                                                       phase_importance_sampling=True))
+                p = dict(p, majorant_layers=pt["majorant_layers"], shell_growth=pt.get("shell_growth", 1.0))
             args.layers = args.layers or p["majorant_layers"]
             args.shell_growth = p.get("shell_growth", 1.0) if args.shell_growth <= 0 else args.shell_growth
             out = bench_batch(args, ctx)

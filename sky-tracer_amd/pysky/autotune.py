@@ -26,7 +26,7 @@ DEFAULT = dict(majorant_layers=16, event_threshold=32, track_steps=8, occupancy=
                pixels_per_item=0, shell_growth=1.0)
 SPACE = {
     "majorant_layers": [2, 3, 4, 5, 6, 7, 8, 12, 16, 24, 32, 64],
-    "shell_growth": [1.0, 1.5, 2.0, 2.5, 3.0, 4.0, 5.0],
+    "shell_growth": [1.0, 1.5, 2.0, 2.5, 3.0, 4.0, 5.0, 6.0, 8.0],
     "event_threshold": [8, 16, 24, 32, 40, 48, 56],
     "track_steps": [1, 2, 4, 8, 16, 32],
     "occupancy": [0, 4, 5, 6, 8],
@@ -41,7 +41,8 @@ def shape_class(scene: SceneData, config: RenderConfig) -> str:
     nb_cls = 3 if nb <= 3 else 4 if nb <= 4 else 8 if nb <= 8 else 16 if nb <= 16 else 32 if nb <= 32 else 64
     px_cls = "small" if npx <= 8192 else "medium" if npx <= 131072 else "large"
     est = f"is{int(config.phase_importance_sampling)}ct{int(config.carried_tau)}rt{int(config.ratio_tracking)}"
-    return f"nl{nb_cls}:{px_cls}:{est}"
+    medium = ":tab" if getattr(scene.atmosphere, "tables", None) is not None else ""
+    return f"nl{nb_cls}:{px_cls}:{est}{medium}"
 
 
 def load_cache() -> Dict[str, dict]:
