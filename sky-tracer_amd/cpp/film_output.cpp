@@ -286,6 +286,7 @@ std::vector<float> read_exr_r(const std::string &path, uint32_t &width, uint32_t
         return std::string((const char *)&b[s], (p++) - s);
     };
     auto i32 = [&](size_t at) {
+        if (at + 4 > b.size()) throw fail("truncated header");
         int32_t v;
         std::memcpy(&v, &b[at], 4);
         return v;
@@ -306,23 +307,28 @@ std::vector<float> read_exr_r(const std::string &path, uint32_t &width, uint32_t
         std::string name = cstr(), type = cstr();
         int32_t size = i32(p);
         p += 4;
+        if (size < 0) throw fail("negative attribute size");
         size_t end = p + (size_t)size;
         if (end > b.size()) throw fail("truncated attribute");
         if (name == "channels") {
             size_t q = p;
             while (q < end && b[q]) {
                 size_t s = q;
-                while (b[q]) ++q;
+                while (q < end && b[q]) ++q;
+                if (q + 17 > end) throw fail("truncated channel list");
                 std::string cn((const char *)&b[s], q - s);
                 ++q;
                 int32_t t;
                 std::memcpy(&t, &b[q], 4);
+                if (t < 0 || t > 2) throw fail("unknown channel type");
                 chans.push_back({cn, t});
                 q += 16;
             }
         } else if (name == "compression") {
+            if (size < 1) throw fail("truncated attribute");
             compression = b[p];
         } else if (name == "dataWindow") {
+            if (size < 16) throw fail("truncated attribute");
             xmin = i32(p);
             ymin = i32(p + 4);
             xmax = i32(p + 8);
@@ -331,8 +337,11 @@ std::vector<float> read_exr_r(const std::string &path, uint32_t &width, uint32_t
         p = end;
     }
     if (compression != 0) throw fail("only uncompressed files are read");
+    if (xmax < xmin || ymax < ymin || (int64_t)xmax - xmin >= (1 << 24) || (int64_t)ymax - ymin >= (1 << 24))
+        throw fail("bad data window");
     width = (uint32_t)(xmax - xmin + 1);
     height = (uint32_t)(ymax - ymin + 1);
+    if (p + 8 * (size_t)height > b.size()) throw fail("truncated offset table");
     size_t pixel_bytes = 0, r_off = (size_t)-1;
     int r_type = -1;
     for (const Ch &c : chans) {
@@ -340,14 +349,15 @@ std::vector<float> read_exr_r(const std::string &path, uint32_t &width, uint32_t
             r_off = pixel_bytes;
             r_type = c.type;
         }
-        pixel_bytes += c.type == 1 ? 2 : 4;
+        pixel_bytes += c.type == 1 ? 2 : 4;  // HALF; UINT and FLOAT are 4 bytes
     }
     if (r_off == (size_t)-1) throw fail("no R channel");
+    if (r_type == 0) throw fail("R channel is UINT");
     std::vector<float> out((size_t)width * height);
     for (uint32_t y = 0; y < height; ++y) {
         uint64_t off;
         std::memcpy(&off, &b[p + 8 * (size_t)y], 8);
-        if (off + 8 + pixel_bytes * width > b.size()) throw fail("truncated scanline");
+        if (off > b.size() || off + 8 + pixel_bytes * width > b.size()) throw fail("truncated scanline");
         size_t line = (size_t)off + 8 + r_off * width;  // channel planes of one line
         for (uint32_t x = 0; x < width; ++x) {
             float v;
@@ -373,14 +383,15 @@ void write_png_rgb8(const std::string &path, uint32_t width, uint32_t height, co
         raw.insert(raw.end(), rgb.begin() + 3 * (size_t)y * width, rgb.begin() + 3 * (size_t)(y + 1) * width);
     }
     std::vector<uint8_t> z = {0x78, 0x01};
-    for (size_t i = 0; i < raw.size() || i == 0; i += 65535) {  // stored deflate blocks
-        size_t n = std::min<size_t>(65535, raw.size() - i);
+    size_t i = 0;
+    do {  // stored deflate blocks of at most 65535 bytes, the last one flagged
+        const size_t n = std::min<size_t>(65535, raw.size() - i);
         z.push_back(i + n >= raw.size() ? 1 : 0);
         put_le<uint16_t>(z, (uint16_t)n);
         put_le<uint16_t>(z, (uint16_t)~n);
         z.insert(z.end(), raw.begin() + i, raw.begin() + i + n);
-        if (raw.empty()) break;
-    }
+        i += n;
+    } while (i < raw.size());
     uint32_t a = 1, bsum = 0;  // Adler-32
     for (uint8_t c : raw) {
         a = (a + c) % 65521u;

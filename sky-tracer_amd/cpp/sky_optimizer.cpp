@@ -66,6 +66,12 @@ private:
     const std::string &s_;
     size_t p_ = 0;
     [[noreturn]] void fail(const char *why) { throw std::runtime_error(std::string("JSON: ") + why); }
+    char peek() const { return p_ < s_.size() ? s_[p_] : '\0'; }
+    char take()
+    {
+        if (p_ >= s_.size()) fail("unexpected end");
+        return s_[p_++];
+    }
     void ws()
     {
         while (p_ < s_.size() && std::strchr(" \t\r\n", s_[p_])) ++p_;
@@ -79,14 +85,13 @@ private:
     }
     std::string string()
     {
-        if (s_[p_] != '"') fail("expected string");
+        if (peek() != '"') fail("expected string");
         ++p_;
         std::string out;
         while (p_ < s_.size() && s_[p_] != '"') {
             char c = s_[p_++];
             if (c == '\\') {
-                if (p_ >= s_.size()) fail("bad escape");
-                char e = s_[p_++];
+                char e = take();
                 switch (e) {
                 case 'n': out += '\n'; break;
                 case 't': out += '\t'; break;
@@ -124,12 +129,12 @@ private:
         ws();
         if (p_ >= s_.size()) fail("unexpected end");
         Json v;
-        char c = s_[p_];
+        char c = peek();
         if (c == '{') {
             v.type = Json::Obj;
             ++p_;
             ws();
-            if (s_[p_] == '}') {
+            if (peek() == '}') {
                 ++p_;
                 return v;
             }
@@ -137,14 +142,14 @@ private:
                 ws();
                 std::string k = string();
                 ws();
-                if (s_[p_++] != ':') fail("expected ':'");
+                if (take() != ':') fail("expected ':'");
                 v.obj.emplace_back(k, value());
                 ws();
-                if (s_[p_] == ',') {
+                if (peek() == ',') {
                     ++p_;
                     continue;
                 }
-                if (s_[p_++] != '}') fail("expected '}'");
+                if (take() != '}') fail("expected '}'");
                 return v;
             }
         }
@@ -152,18 +157,18 @@ private:
             v.type = Json::Arr;
             ++p_;
             ws();
-            if (s_[p_] == ']') {
+            if (peek() == ']') {
                 ++p_;
                 return v;
             }
             for (;;) {
                 v.arr.push_back(value());
                 ws();
-                if (s_[p_] == ',') {
+                if (peek() == ',') {
                     ++p_;
                     continue;
                 }
-                if (s_[p_++] != ']') fail("expected ']'");
+                if (take() != ']') fail("expected ']'");
                 return v;
             }
         }
