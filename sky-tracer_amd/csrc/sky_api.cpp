@@ -130,7 +130,7 @@ sky_status run_render(sky_context *ctx, sky::HostScene &hs, hipStream_t stream,
         j.t_done = (unsigned long long *)((uint32_t *)ctx->meta.p + 6);
         HIP_TRY(hipMemcpyAsync(ctx->jobs.p, &j, sizeof(j), hipMemcpyHostToDevice, stream), "upload job",
                 SKY_ERR_COPY);
-        plan = sky::plan_launch(ctx->tuning, ctx->num_cus, j.n_items, s.nlam * s.L, s.L);
+        plan = sky::plan_launch(ctx->tuning, ctx->num_cus, j.n_items, s.nlam * s.L, s.L, s.flags);
     } else if (!s.film) {
         size_t bytes = (size_t)npx * s.nlam * sizeof(float);
         HIP_TRY(ctx->film.ensure(bytes), "alloc film", SKY_ERR_OOM);
@@ -444,7 +444,7 @@ sky_status adjoint_impl(sky_context *ctx, const sky_planet *planet, const sky_at
     j.done = (uint32_t *)ctx->meta.p + 4;
     j.t_done = (unsigned long long *)((uint32_t *)ctx->meta.p + 6);
     HIP_TRY(hipMemcpyAsync(ctx->jobs.p, &j, sizeof(j), hipMemcpyHostToDevice, stream), "upload job", SKY_ERR_COPY);
-    sky::WavePlan plan = sky::plan_launch(ctx->tuning, ctx->num_cus, j.n_items, s.nlam * s.L, s.L);
+    sky::WavePlan plan = sky::plan_launch(ctx->tuning, ctx->num_cus, j.n_items, s.nlam * s.L, s.L, sky::kFlagMixed);
     sky::AdjArgs A;
     A.dl_dxyz = d_dl;
     A.lam_units = (const float4 *)ctx->adj_units.p;
@@ -548,7 +548,7 @@ sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint3
 
     // Table offsets.
     size_t n_lam = 0, n_maj = 0, n_shl = 0, n_xyz = 0, n_film = 0, n_acc = 0, n_tab = 0;
-    uint32_t max_npx = 0, total_items = 0, max_maj = 0, max_layers = 0;
+    uint32_t max_npx = 0, total_items = 0, max_maj = 0, max_layers = 0, flags_all = 0;
     std::vector<size_t> o_lam(n), o_maj(n), o_shl(n), o_xyz(n), o_film(n), o_acc(n), o_tab(n);
     ctx->host_jobs.assign(n, sky::DevJob{});
     const uint32_t min_items = 0;  // the batch balances across renders; no sample split needed
@@ -579,6 +579,7 @@ sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint3
         max_npx = npx > max_npx ? npx : max_npx;
         max_maj = h.maj.size() > max_maj ? (uint32_t)h.maj.size() : max_maj;
         max_layers = h.dev.L > max_layers ? h.dev.L : max_layers;
+        flags_all = k == 0 || h.dev.flags == flags_all ? h.dev.flags : sky::kFlagMixed;
         samples += (uint64_t)npx * h.dev.spp * h.dev.nlam;
     }
     const uint32_t meta_words = 8 + 4 * n;
@@ -633,7 +634,7 @@ sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint3
         HIP_TRY(hipMemcpyAsync(ctx->b_tab.p, ctx->host_btab.data(), n_tab, hipMemcpyHostToDevice, stream),
                 "upload batch medium tables", SKY_ERR_COPY);
     HIP_TRY(sky::launch_zero((uint32_t *)ctx->diag.p, 16, stream), "clear diagnostics", SKY_ERR_SYNC);
-    sky::WavePlan plan = sky::plan_launch(ctx->tuning, ctx->num_cus, total_items, max_maj, max_layers);
+    sky::WavePlan plan = sky::plan_launch(ctx->tuning, ctx->num_cus, total_items, max_maj, max_layers, flags_all);
 
     // The graph holds only kernel nodes (zero, integrate, finalize); the
     // tables it reads are uploaded above on every call.
@@ -650,6 +651,7 @@ sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint3
     mix(plan.track_steps);
     mix(plan.occupancy);
     mix(plan.lds_bytes);
+    mix(plan.fast);
     bool reused = false;
     HIP_TRY(hipEventRecord(ctx->ev0, stream), "event record", SKY_ERR_SYNC);
     if (use_graph) {

@@ -338,6 +338,24 @@ __device__ __forceinline__ float next_crossing(const DevScene &s, const ShellTab
     return f_sqrt((float)(disc > 0.0 ? disc : 0.0)) - bf;
 }
 
+// The same crossing without branches, for a shell table in LDS: both
+// boundaries of shell k are read at once and the root of the selected one is
+// taken, so every lane of a wave runs the same instructions.  Bit-identical
+// to next_crossing.
+__device__ __forceinline__ float next_crossing_sel(const DevScene &s, LdsDoublePtr tab, double q, float bf, int k,
+                                                   int out, int &kn, int &on)
+{
+    const double din = tab[k] - q;       // inner boundary of shell k
+    const double dout = tab[k + 1] - q;  // outer boundary (the table has L + 1 entries)
+    const bool inw = !out && din > 0.0;
+    const bool edge = inw ? k == 0 : k + 1 >= (int)s.L;
+    on = inw ? 0 : 1;
+    kn = edge ? (inw ? 0 : k) : (inw ? k - 1 : k + 1);
+    const float r = f_sqrt((float)(inw ? din : (dout > 0.0 ? dout : 0.0)));
+    const float tc = (inw ? -r : r) - bf;
+    return edge ? __builtin_inff() : tc;
+}
+
 __device__ __forceinline__ float next_crossing(const DevScene &s, const ShellTab &tab, const Ray &r, int k,
                                                int out, int &kn, int &on)
 {

@@ -102,15 +102,20 @@ struct WavePlan {
     uint32_t maj_lds = 0;   // floats per wave for the majorant grid in LDS (0: global)
     uint32_t shell_lds = 0; // doubles per wave for the shell table in LDS (0 with maj_lds 0)
     uint32_t lds_bytes = 0; // dynamic LDS per block
+    bool fast = false;      // the _fast kernels (see plan_launch)
 };
 
 void plan_job(const DevScene &s, const sky_tuning &t, int num_cus, uint32_t min_items, DevJob &j);
 size_t job_accum_bytes(const DevJob &j);
 uint32_t launch_min_items(const sky_tuning &t, int num_cus);
 // max_maj: the largest nlam x majorant_layers over the launch's jobs;
-// max_layers: the largest majorant_layers.
+// max_layers: the largest majorant_layers; flags_all: the estimator flags if
+// every job has the same ones, else kFlagMixed.  The _fast kernels serve
+// carried optical depth + lobe sampling + delta-tracked shadow rays with the
+// tables in LDS.
+constexpr uint32_t kFlagMixed = 0x80000000u;
 WavePlan plan_launch(const sky_tuning &t, int num_cus, uint32_t total_items, uint32_t max_maj,
-                     uint32_t max_layers);
+                     uint32_t max_layers, uint32_t flags_all);
 hipError_t launch_pool(const DevJob *d_jobs, uint32_t n_jobs, uint32_t max_npx, const WavePlan &p,
                        uint32_t *meta, uint32_t meta_words, hipStream_t stream);
 hipError_t launch_zero(uint32_t *p, uint32_t n, hipStream_t stream);

@@ -81,7 +81,7 @@ struct LaneCtx {
     float t, tau, seg_end, maj;
     float aux;  // shadow-ray transmittance (0/1 delta, product for ratio tracking)
     int k, out, kn, on;
-    uint32_t guard;
+    uint32_t guard;  // shell-crossing watchdog (see ray_begin)
 };
 
 // ------------------------------------------------------------ adjoint
@@ -211,15 +211,24 @@ __device__ __forceinline__ float majorant(const MajSrc &m, uint32_t L, uint32_t 
     return m.glob[i];
 }
 
+// FAST: the launch holds every job's tables in LDS (no global fallback).
+template <bool FAST>
 __device__ __forceinline__ void seg_begin(const DevScene &s, const MajSrc &m, LaneCtx &c)
 {
-    float tc = next_crossing(s, m.shells, c.q, c.bf, c.k, c.out, c.kn, c.on);
+    float tc;
+    if constexpr (FAST) {
+        tc = next_crossing_sel(s, m.shells.lds, c.q, c.bf, c.k, c.out, c.kn, c.on);
+        c.maj = m.lds[c.band * s.L + (uint32_t)c.k];
+    } else {
+        tc = next_crossing(s, m.shells, c.q, c.bf, c.k, c.out, c.kn, c.on);
+        c.maj = majorant(m, s.L, c.band, c.k);
+    }
     c.seg_end = fmaxf(fminf(tc, c.tmax), c.t);
-    c.maj = majorant(m, s.L, c.band, c.k);
 }
 
 // Sets up a ray from (ox, oy, oz) along d (FP64 set-up, sky_device.hpp) and
 // starts tracking it; returns false if it has no boundary.
+template <bool FAST>
 __device__ __forceinline__ bool ray_begin(const DevScene &s, const MajSrc &m, LaneCtx &c, double ox,
                                           double oy, double oz, f3 d, int mode)
 {
@@ -235,10 +244,11 @@ __device__ __forceinline__ bool ray_begin(const DevScene &s, const MajSrc &m, La
     c.t = 0.0f;
     c.tau = -1.0f;
     c.aux = 1.0f;
-    c.guard = 1;
+    // crossings made so far (from 1), or in the FAST walk the crossings left
+    c.guard = FAST ? s.max_segments : 1u;
     c.k = r.layer;
     c.out = r.outward;
-    seg_begin(s, m, c);
+    seg_begin<FAST>(s, m, c);
     c.st = ST_TRACK;
     return r.hit != 0;
 }
@@ -258,12 +268,53 @@ __device__ __forceinline__ Coeffs lane_medium(const DevScene &s, const LaneCtx &
 // carried optical depth, a crossing uses up part of the current draw) until
 // a tentative collision lies inside the current segment or the ray ends.
 // The collision is then evaluated and accepted or rejected in the same step.
-template <bool ADJ>
+// FAST (carried optical depth, lobe sampling, delta-tracked shadow rays,
+// tables in LDS) draws the step's free flight before the walk: the first
+// draw of a step is made in its first non-empty segment either way, so the
+// random sequence is unchanged, and the walk loop is left with crossings only.
+template <bool ADJ, bool FAST>
 __device__ __forceinline__ void track_step(const DevScene &s, const MajSrc &m, LaneCtx &c, AdjLane &a,
                                            const AdjArgs &A, bool &wd)
 {
-    const bool carried = (s.flags & kFlagCarriedTau) != 0;
     float tn = 0.0f;
+    if constexpr (FAST) {
+        // Each trip either finds the collision inside the current segment or
+        // crosses into the next one, with selects in place of branches and
+        // one exit per outcome; same arithmetic and draw order as below.
+        if (c.tau < 0.0f && c.t < c.seg_end) c.tau = free_flight(rng_next(c.rng));
+        bool ended = false;
+        for (;;) {
+            const bool nonempty = c.t < c.seg_end;
+            const float seg_tau = c.maj * (c.seg_end - c.t);
+            const bool cross = !nonempty || c.tau >= seg_tau;
+            tn = c.t + f_div(c.tau, c.maj);
+            if (!cross && tn < c.seg_end) break;
+            // a draw that lands at or past the segment end by rounding is spent
+            c.tau = cross ? (nonempty ? c.tau - seg_tau : c.tau) : -1.0f;
+            c.t = c.seg_end;
+            if ((c.t >= c.tmax) | (c.guard == 0u)) {
+                ended = true;
+                break;
+            }
+            c.guard--;
+            c.k = c.kn;
+            c.out = c.on;
+            seg_begin<FAST>(s, m, c);
+            if (c.tau < 0.0f && c.t < c.seg_end) c.tau = free_flight(rng_next(c.rng));
+        }
+        if (ended) {
+            c.st = ST_EVENT;
+            c.ev = c.mode == kTrackCollision ? EV_BOUNDARY : EV_SHADOW;
+            if (c.t < c.tmax) {
+                wd = true;
+                c.aux = 0.0f;
+                c.kind = 2;  // watchdog: ends the path without contribution
+            }
+            return;
+        }
+        c.tau = -1.0f;
+    } else {
+    const bool carried = (s.flags & kFlagCarriedTau) != 0;
     for (;;) {
         if (c.t < c.seg_end) {
             if (c.tau < 0.0f) c.tau = free_flight(rng_next(c.rng));
@@ -293,14 +344,15 @@ __device__ __forceinline__ void track_step(const DevScene &s, const MajSrc &m, L
         c.guard++;
         c.k = c.kn;
         c.out = c.on;
-        seg_begin(s, m, c);
+        seg_begin<FAST>(s, m, c);
+    }
     }
     c.t = tn;
     const float h = lane_altitude(s, c, tn);
     Coeffs co = lane_medium(s, c, h);
     float st = co.sr + co.sas + co.so + co.saa;
     float ratio = fminf(fmaxf(f_div(st, c.maj), 0.0f), 1.0f);
-    if (c.mode == kTrackRatio) {
+    if (!FAST && c.mode == kTrackRatio) {
         c.aux *= 1.0f - ratio;
         if constexpr (ADJ) {
             float f = -1.0f / fmaxf(c.maj - st, 1e-30f);
@@ -377,13 +429,13 @@ __device__ __forceinline__ void finish_path(LaneCtx &c, AdjLane &a, float *acc_s
 // the ray set-up: it leaves the lane in RAY (c.d, c.mode and, for a shadow
 // ray, c.nee_factor set), EVENT (a zero sun sample, for the next pass) or
 // NEED (path finished and accumulated).
-template <bool ADJ>
+template <bool ADJ, bool FAST>
 __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m, LaneCtx &c, AdjLane &a,
                                              const AdjArgs &A, float *acc_sum, float *acc_sq, float *acc_grad,
                                              bool &wd)
 {
     const uint32_t n_light = s.dls;
-    const bool phase_is = (s.flags & kFlagPhaseIS) != 0;
+    const bool phase_is = FAST || (s.flags & kFlagPhaseIS) != 0;
     int nx = NX_FINISH;
 
     // 1. the event; a ground hit or an accepted collision moves the vertex
@@ -498,7 +550,7 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
                 add_dlog_phase(s, A, a, c.band, mu, c.bo, a.SS);
             }
         }
-        mode = (s.flags & kFlagRatio) ? kTrackRatio : kTrackDelta;
+        mode = (!FAST && (s.flags & kFlagRatio)) ? kTrackRatio : kTrackDelta;
         if (!need_ray) {
             // ground facing away from the sun: a zero sample
             c.st = ST_EVENT;
@@ -580,12 +632,13 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
 // main ray from the current origin (vertex already moved, or the camera).
 // A shadow ray that does not leave the atmosphere is a zero sun sample; a
 // main ray with no boundary ends the path at the next pass.
+template <bool FAST>
 __device__ __forceinline__ void ray_pass(const DevScene &s, const MajSrc &m, LaneCtx &c)
 {
     const bool shadow = c.mode != kTrackCollision;
     const f3 dir = c.d;
     const double e = shadow ? (double)kRayEpsilonKm : 0.0;
-    bool hit = ray_begin(s, m, c, c.px + (double)dir.x * e, c.py + (double)dir.y * e, c.pz + (double)dir.z * e, dir,
+    bool hit = ray_begin<FAST>(s, m, c, c.px + (double)dir.x * e, c.py + (double)dir.y * e, c.pz + (double)dir.z * e, dir,
                          c.mode);
     if (shadow) {
         if (!(hit && c.kind == 1)) {
@@ -685,7 +738,7 @@ __device__ __forceinline__ uint32_t find_job(const PoolArgs &a, uint32_t item)
     return __builtin_amdgcn_readfirstlane(lo);
 }
 
-template <bool ADJ>
+template <bool ADJ, bool FAST>
 __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &A)
 {
     extern __shared__ float dyn_lds[];
@@ -735,7 +788,7 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
         msrc.lds = (LdsFloatPtr)lds_maj;
         msrc.glob = (GlobalFloatPtr)s.maj;
         const uint32_t nmaj = s.nlam * s.L;
-        msrc.use_lds = nmaj <= a.maj_lds && s.L + 1 <= a.shell_lds;
+        msrc.use_lds = FAST || (nmaj <= a.maj_lds && s.L + 1 <= a.shell_lds);
         msrc.shells.lds = (LdsDoublePtr)lds_shell;
         msrc.shells.glob = (GlobalDoublePtr)s.shell_d;
         msrc.shells.use_lds = msrc.use_lds;
@@ -760,7 +813,7 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
             if (!evm && !trm && !ndm) break;
             if ((evm || ndm) && ((uint32_t)__popcll(evm | ndm) >= a.event_threshold || !trm)) {
                 // handler pass: events, refill of finished lanes, ray set-ups
-                if (c.st == ST_EVENT) handle_event<ADJ>(s, msrc, c, al, A, acc_sum, acc_sq, acc_grad, wd);
+                if (c.st == ST_EVENT) handle_event<ADJ, FAST>(s, msrc, c, al, A, acc_sum, acc_sq, acc_grad, wd);
                 const uint64_t need = ballot(c.st == ST_NEED);
                 if (need && next < total) {
                     if (c.st == ST_NEED) {
@@ -772,11 +825,11 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
                     }
                     next += (uint32_t)__popcll(need);
                 }
-                if (c.st == ST_RAY) ray_pass(s, msrc, c);
+                if (c.st == ST_RAY) ray_pass<FAST>(s, msrc, c);
             }
             for (uint32_t i = 0; i < a.track_steps; ++i) {
                 if (!ballot(c.st == ST_TRACK)) break;
-                if (c.st == ST_TRACK) track_step<ADJ>(s, msrc, c, al, A, wd);
+                if (c.st == ST_TRACK) track_step<ADJ, FAST>(s, msrc, c, al, A, wd);
             }
         }
         __builtin_amdgcn_wave_barrier();
@@ -799,14 +852,22 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
 // The same integrator under several register budgets: the compiler's own
 // allocation and caps for at least 4, 5, 6 or 8 waves per SIMD (512 VGPRs per
 // lane / waves, 8-register granules: 128, 96, 80, 64).  The autotuner picks
-// one per workload.
-__global__ __launch_bounds__(256) void k_wavepool(PoolArgs a) { wavepool_body<false>(a, AdjArgs{}); }
+// one per workload.  The _fast kernels serve launches whose jobs all use
+// carried optical depth, lobe sampling and delta-tracked shadow rays with
+// their tables in LDS.
+__global__ __launch_bounds__(256) void k_wavepool(PoolArgs a) { wavepool_body<false, false>(a, AdjArgs{}); }
+__global__ __launch_bounds__(256) void k_wavepool_fast(PoolArgs a) { wavepool_body<false, true>(a, AdjArgs{}); }
 
-#define SKY_WAVEPOOL_OCC(N)                                                                          \
-    __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N))) void k_wavepool_o##N( \
-        PoolArgs a)                                                                                  \
-    {                                                                                                \
-        wavepool_body<false>(a, AdjArgs{});                                                          \
+#define SKY_WAVEPOOL_OCC(N)                                                                               \
+    __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N))) void k_wavepool_o##N(      \
+        PoolArgs a)                                                                                       \
+    {                                                                                                     \
+        wavepool_body<false, false>(a, AdjArgs{});                                                        \
+    }                                                                                                     \
+    __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N))) void k_wavepool_fast_o##N( \
+        PoolArgs a)                                                                                       \
+    {                                                                                                     \
+        wavepool_body<false, true>(a, AdjArgs{});                                                         \
     }
 SKY_WAVEPOOL_OCC(4)
 SKY_WAVEPOOL_OCC(5)
@@ -815,7 +876,7 @@ SKY_WAVEPOOL_OCC(8)
 #undef SKY_WAVEPOOL_OCC
 
 // Path-replay adjoint: the forward state machine plus the score bookkeeping.
-__global__ __launch_bounds__(256) void k_wavepool_adjoint(PoolArgs a, AdjArgs A) { wavepool_body<true>(a, A); }
+__global__ __launch_bounds__(256) void k_wavepool_adjoint(PoolArgs a, AdjArgs A) { wavepool_body<true, false>(a, A); }
 
 // Per job (blockIdx.y): reduces the chunk partials in chunk order,
 // normalises by 1/spp and clamps (integrator.rs:307-310), and forms
@@ -891,10 +952,12 @@ size_t job_accum_bytes(const DevJob &j)
 }
 
 WavePlan plan_launch(const sky_tuning &t, int num_cus, uint32_t total_items, uint32_t max_maj,
-                     uint32_t max_layers)
+                     uint32_t max_layers, uint32_t flags_all)
 {
     WavePlan p;
     p.maj_lds = max_maj <= kMajLdsMax ? (max_maj + 63u) & ~63u : 0u;
+    const uint32_t fast_flags = kFlagCarriedTau | kFlagPhaseIS;
+    p.fast = p.maj_lds != 0 && (flags_all & (fast_flags | kFlagRatio)) == fast_flags;
     p.shell_lds = p.maj_lds ? ((max_layers + 1u + 7u) & ~7u) : 0u;
     p.lds_bytes = kWavesPerBlock * (kLdsFixed + p.maj_lds + 2 * p.shell_lds) * (uint32_t)sizeof(float);
     uint32_t wpc = t.waves_per_cu ? t.waves_per_cu : 16u;
@@ -931,13 +994,16 @@ hipError_t launch_pool(const DevJob *d_jobs, uint32_t n_jobs, uint32_t max_npx, 
     a.track_steps = p.track_steps;
     a.maj_lds = p.maj_lds;
     a.shell_lds = p.shell_lds;
+    typedef void (*PoolKernel)(PoolArgs);
+    PoolKernel k;
     switch (p.occupancy) {
-    case 4: hipLaunchKernelGGL(k_wavepool_o4, dim3(p.grid), dim3(p.block), p.lds_bytes, stream, a); break;
-    case 5: hipLaunchKernelGGL(k_wavepool_o5, dim3(p.grid), dim3(p.block), p.lds_bytes, stream, a); break;
-    case 6: hipLaunchKernelGGL(k_wavepool_o6, dim3(p.grid), dim3(p.block), p.lds_bytes, stream, a); break;
-    case 8: hipLaunchKernelGGL(k_wavepool_o8, dim3(p.grid), dim3(p.block), p.lds_bytes, stream, a); break;
-    default: hipLaunchKernelGGL(k_wavepool, dim3(p.grid), dim3(p.block), p.lds_bytes, stream, a); break;
+    case 4: k = p.fast ? k_wavepool_fast_o4 : k_wavepool_o4; break;
+    case 5: k = p.fast ? k_wavepool_fast_o5 : k_wavepool_o5; break;
+    case 6: k = p.fast ? k_wavepool_fast_o6 : k_wavepool_o6; break;
+    case 8: k = p.fast ? k_wavepool_fast_o8 : k_wavepool_o8; break;
+    default: k = p.fast ? k_wavepool_fast : k_wavepool; break;
     }
+    hipLaunchKernelGGL(k, dim3(p.grid), dim3(p.block), p.lds_bytes, stream, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_finalize, dim3((max_npx + 255) / 256, n_jobs), dim3(256), 0, stream, d_jobs);
