@@ -264,6 +264,7 @@ struct Ray {
 // and the initial shell of the walk, found by comparing |o|^2 - R^2 with the
 // shell table (layer_for_altitude, wgsl:252-258, generalised to non-uniform
 // shells).  d is an FP32-normalised direction.
+template <int COUNT_L = 0>
 __device__ __forceinline__ void ray_setup(const DevScene &s, const ShellTab &tab, double ox, double oy,
                                           double oz, f3 d, Ray &r)
 {
@@ -300,12 +301,21 @@ __device__ __forceinline__ void ray_setup(const DevScene &s, const ShellTab &tab
     }
     const double w0 = oo - s.R * s.R;
     const int L = (int)s.L;
-    const float wf = (float)w0;
-    const float h0 = f_div(wf, f_sqrt(fmaxf(s.R2f + wf, 0.0f)) + s.Rf);
-    float kg = s.shell_c1 > 0.0f ? f_log2(fmaxf(1.0f + h0 * s.shell_c1, 1.0f)) * s.shell_c2 : h0 * s.shell_c2;
-    int k = (int)fminf(fmaxf(kg, 0.0f), (float)(L - 1));
-    while (k > 0 && tab[k] > w0) --k;
-    while (k < L - 1 && tab[k + 1] <= w0) ++k;
+    int k = 0;
+    if constexpr (COUNT_L > 0) {
+        // at most COUNT_L shells (L <= COUNT_L): the shell holding the origin
+        // is the number of interior boundaries at or below it
+#pragma unroll
+        for (int j = 1; j < COUNT_L; ++j)
+            if (j < L) k += tab[j] <= w0 ? 1 : 0;
+    } else {
+        const float wf = (float)w0;
+        const float h0 = f_div(wf, f_sqrt(fmaxf(s.R2f + wf, 0.0f)) + s.Rf);
+        float kg = s.shell_c1 > 0.0f ? f_log2(fmaxf(1.0f + h0 * s.shell_c1, 1.0f)) * s.shell_c2 : h0 * s.shell_c2;
+        k = (int)fminf(fmaxf(kg, 0.0f), (float)(L - 1));
+        while (k > 0 && tab[k] > w0) --k;
+        while (k < L - 1 && tab[k + 1] <= w0) ++k;
+    }
     r.outward = b >= 0.0;
     if (!r.outward && k > 0 && w0 - tab[k] <= s.shell_tol) k -= 1;
     r.layer = k;

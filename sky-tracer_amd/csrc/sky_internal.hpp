@@ -112,8 +112,9 @@ uint32_t launch_min_items(const sky_tuning &t, int num_cus);
 // max_layers: the largest majorant_layers; flags_all: the estimator flags if
 // every job has the same ones, else kFlagMixed.  The _fast kernels serve
 // carried optical depth + lobe sampling + delta-tracked shadow rays with the
-// tables in LDS.
+// tables in LDS and at most kFastMaxShells majorant shells.
 constexpr uint32_t kFlagMixed = 0x80000000u;
+constexpr uint32_t kFastMaxShells = 8;
 WavePlan plan_launch(const sky_tuning &t, int num_cus, uint32_t total_items, uint32_t max_maj,
                      uint32_t max_layers, uint32_t flags_all);
 hipError_t launch_pool(const DevJob *d_jobs, uint32_t n_jobs, uint32_t max_npx, const WavePlan &p,

@@ -233,7 +233,7 @@ __device__ __forceinline__ bool ray_begin(const DevScene &s, const MajSrc &m, La
                                           double oy, double oz, f3 d, int mode)
 {
     Ray r;
-    ray_setup(s, m.shells, ox, oy, oz, d, r);
+    ray_setup<FAST ? (int)kFastMaxShells : 0>(s, m.shells, ox, oy, oz, d, r);
     c.d = d;
     c.bf = r.bf;
     c.qf = r.qf;
@@ -406,6 +406,17 @@ __device__ __forceinline__ void advance_vertex(LaneCtx &c, float t)
 
 enum Next : int { NX_NEE = 1, NX_CONT = 2, NX_MAIN = 3, NX_FINISH = 4 };
 
+// Weight divisions of the event handler: the FAST kernels multiply by the
+// hardware reciprocal (within 1 ulp of the quotient), the others divide.
+template <bool FAST>
+__device__ __forceinline__ float wdiv(float a, float b)
+{
+    if constexpr (FAST)
+        return f_div(a, b);
+    else
+        return a / b;
+}
+
 // Path end: the path's radiance and its square into the wave's LDS sums.
 template <bool ADJ>
 __device__ __forceinline__ void finish_path(LaneCtx &c, AdjLane &a, float *acc_sum, float *acc_sq,
@@ -534,7 +545,7 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
     if (nx == NX_NEE) {
         if (c.vtype == 1) {
             float cs = fmaxf(dot3(nrm, dir), 0.0f);
-            factor = s.albedo * kInvPi * c.lsun * cs / s.cone_pdf;
+            factor = wdiv<FAST>(s.albedo * kInvPi * c.lsun * cs, s.cone_pdf);
             need_ray = cs > 0.0f;
             if constexpr (ADJ) {
                 for (int k = 0; k < kNP; ++k) a.SS[k] = a.S[k];
@@ -543,7 +554,7 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
         } else {
             float mu = dot3(c.vin, dir);
             float ph = c.rw * phase_rayleigh(mu) + (1.0f - c.rw) * phase_hg(s.g, mu);
-            factor = c.lsun * ph / s.cone_pdf;
+            factor = wdiv<FAST>(c.lsun * ph, s.cone_pdf);
             need_ray = true;
             if constexpr (ADJ) {
                 for (int k = 0; k < kNP; ++k) a.SS[k] = a.S[k];
@@ -559,7 +570,7 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
             c.aux = 0.0f;
         }
     } else if (nx == NX_CONT) {
-        c.L = c.L + c.tp * (c.nee_sum / (float)n_light);
+        c.L = c.L + c.tp * wdiv<FAST>(c.nee_sum, (float)n_light);
         float w;
         bool ok = true;
         if (c.vtype == 0) {
@@ -586,7 +597,7 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
                 if (urr > q)
                     live = false;
                 else
-                    c.tp = c.tp / q;
+                    c.tp = wdiv<FAST>(c.tp, q);
             }
             if (live) {
                 if (c.L < 0.0f || c.L != c.L || c.tp < 0.0f || c.tp != c.tp) {
@@ -957,7 +968,7 @@ WavePlan plan_launch(const sky_tuning &t, int num_cus, uint32_t total_items, uin
     WavePlan p;
     p.maj_lds = max_maj <= kMajLdsMax ? (max_maj + 63u) & ~63u : 0u;
     const uint32_t fast_flags = kFlagCarriedTau | kFlagPhaseIS;
-    p.fast = p.maj_lds != 0 && (flags_all & (fast_flags | kFlagRatio)) == fast_flags;
+    p.fast = p.maj_lds != 0 && max_layers <= kFastMaxShells && (flags_all & (fast_flags | kFlagRatio)) == fast_flags;
     p.shell_lds = p.maj_lds ? ((max_layers + 1u + 7u) & ~7u) : 0u;
     p.lds_bytes = kWavesPerBlock * (kLdsFixed + p.maj_lds + 2 * p.shell_lds) * (uint32_t)sizeof(float);
     uint32_t wpc = t.waves_per_cu ? t.waves_per_cu : 16u;
