@@ -638,6 +638,128 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
     if (nx == NX_FINISH) finish_path<ADJ>(c, a, acc_sum, acc_sq, acc_grad);
 }
 
+// handle_event for the FAST kernels (lobe sampling, delta-tracked shadow
+// rays, no adjoint): the same steps with every branch that has no side effect
+// turned into selects, so a pass costs one evaluation of each piece instead of
+// the exec-mask bookkeeping of nested branches.  Same draws, same arithmetic.
+__device__ __forceinline__ void handle_event_fast(const DevScene &s, LaneCtx &c, float *acc_sum, float *acc_sq,
+                                                  bool &wd)
+{
+    const uint32_t n_light = s.dls;
+
+    // 1. the event
+    const bool bnd = c.ev == EV_BOUNDARY, hit = c.ev == EV_HIT, shd = !bnd && !hit;
+    const bool at_ground = bnd && c.kind == 0;
+    {
+        // advance_vertex for a ground hit or an accepted collision
+        const double dx = c.d.x, dy = c.d.y, dz = c.d.z;
+        const double f = inv_norm64(dx, dy, dz) * (double)(at_ground ? c.tmax : c.t);
+        const bool mv = at_ground || hit;
+        c.px = mv ? c.px + dx * f : c.px;
+        c.py = mv ? c.py + dy * f : c.py;
+        c.pz = mv ? c.pz + dz * f : c.pz;
+    }
+    // exit: the sun disk is seen only by the camera ray (wgsl:745-749)
+    const bool sun_seen = bnd && c.kind == 1 && c.depth == 0 && in_sun_cone(s, c.d);
+    c.L = sun_seen ? c.L + c.tp * c.lsun : c.L;
+    wd = wd || (bnd && c.kind == 3);
+    const bool scatter_vertex = hit && c.tp > 0.0f;
+    const bool nee_start = at_ground || scatter_vertex;
+    c.vtype = at_ground ? 1 : (scatter_vertex ? 0 : c.vtype);
+    c.vin.x = scatter_vertex ? c.d.x : c.vin.x;
+    c.vin.y = scatter_vertex ? c.d.y : c.vin.y;
+    c.vin.z = scatter_vertex ? c.d.z : c.vin.z;
+    // a shadow result adds its sun sample
+    const float nee_sum = shd ? c.nee_sum + c.nee_factor * c.aux : c.nee_sum;
+    const uint32_t nee_i = shd ? c.nee_i + 1u : c.nee_i;
+    c.nee_sum = nee_start ? 0.0f : nee_sum;
+    c.nee_i = nee_start ? 0u : nee_i;
+    int nx = nee_start ? NX_NEE : (shd ? (nee_i < n_light ? NX_NEE : NX_CONT) : NX_FINISH);
+    const bool is_nee = nx == NX_NEE, is_cont = nx == NX_CONT;
+    const bool scat_v = c.vtype == 0;
+
+    // 2. this pass's random numbers in the reference's order: sun cone (2);
+    //    lobe choice, mu, azimuth (3) or cosine lobe (2), then the roulette draw
+    const bool rr = c.depth >= s.rr_start;
+    const uint32_t ndraw = is_nee ? 2u : (is_cont ? (scat_v ? 3u : 2u) + (rr ? 1u : 0u) : 0u);
+    const float u0 = rng_peek(c.rng, 0);
+    const float u1 = rng_peek(c.rng, 1);
+    float u2 = 0.0f, u3 = 0.0f;
+    if (ndraw > 2) u2 = rng_peek(c.rng, 2);
+    if (ndraw > 3) u3 = rng_peek(c.rng, 3);
+    c.rng.w += ndraw * kLcgMul;
+
+    // 3. one direction: sun cone (NEE), Rayleigh/HG lobe at a scattering
+    //    vertex, cosine lobe at the ground
+    const f3 nrm = normalize_or_y(f3{(float)c.px, (float)c.py, (float)c.pz});
+    float mu_sun, st_sun;
+    sun_cone_mu(s, u0, mu_sun, st_sun);
+    const float mu_lobe = u0 < c.rw ? sample_rayleigh_mu(u1) : sample_hg_mu(s.g, u1);
+    const float st_lobe = f_sqrt(fmaxf(1.0f - mu_lobe * mu_lobe, 0.0f));
+    const float mu_cos = f_sqrt(fmaxf(1.0f - u0, 0.0f)), st_cos = f_sqrt(u0);
+    const float mu = is_nee ? mu_sun : (scat_v ? mu_lobe : mu_cos);
+    const float st = is_nee ? st_sun : (scat_v ? st_lobe : st_cos);
+    const float phi = (!is_nee && scat_v) ? u2 : u1;
+    const f3 sun{s.sun[0], s.sun[1], s.sun[2]};
+    const f3 axis = is_nee ? sun : (scat_v ? c.vin : (c.vtype == 1 ? nrm : f3{0.0f, 1.0f, 0.0f}));
+    const f3 dir = (is_nee || is_cont) ? frame_dir(axis, mu, st, phi) : c.vin;
+
+    // 4. weights.  NEE: the sun sample's factor; a ground facing away from
+    //    the sun is a zero sample without a ray.
+    const f3 gn = c.vtype == 1 ? nrm : f3{0.0f, 1.0f, 0.0f};
+    const float cs = fmaxf(dot3(gn, dir), 0.0f);
+    const float mu_v = dot3(c.vin, dir);
+    const float ph = c.rw * phase_rayleigh(mu_v) + (1.0f - c.rw) * phase_hg(s.g, mu_v);
+    const float factor = wdiv<true>(c.vtype == 1 ? s.albedo * kInvPi * c.lsun * cs : c.lsun * ph, s.cone_pdf);
+    const bool nee_ray = c.vtype == 1 ? cs > 0.0f : true;
+    // CONT: the NEE average, the bounce weight (1 for lobe sampling, albedo
+    // for the cosine lobe) and the roulette (wgsl:691-701, 751-770)
+    const float Lc = c.L + c.tp * wdiv<true>(c.nee_sum, (float)n_light);
+    const bool ok = scat_v || cs > 0.0f;
+    const float tp1 = c.tp * (scat_v ? 1.0f : s.albedo);
+    const float q = fminf(fmaxf(tp1, 0.05f), 0.95f);
+    const bool killed = rr && (scat_v ? u3 : u2) > q;
+    const float tp2 = rr ? wdiv<true>(tp1, q) : tp1;
+    const bool live = ok && !killed;
+    const bool bad = Lc < 0.0f || Lc != Lc || tp2 < 0.0f || tp2 != tp2;
+    const bool bounce = is_cont && live && !bad;
+    if (is_cont) {
+        c.L = live && bad ? 0.0f : Lc;
+        c.tp = ok ? (killed ? tp1 : tp2) : c.tp;
+    }
+    c.depth = bounce ? c.depth + 1u : c.depth;
+    c.px = bounce ? c.px + (double)dir.x * kRayEpsilonKm : c.px;
+    c.py = bounce ? c.py + (double)dir.y * kRayEpsilonKm : c.py;
+    c.pz = bounce ? c.pz + (double)dir.z * kRayEpsilonKm : c.pz;
+    nx = is_cont ? (bounce ? NX_MAIN : NX_FINISH) : nx;
+    // loop top of trace_path (wgsl:709-717) plus the depth cap
+    bool main_ray = false;
+    if (nx == NX_MAIN) {
+        const bool capped = s.max_depth && c.depth >= s.max_depth;
+        const bool dog = !capped && c.depth >= s.watchdog_limit;
+        wd = wd || dog;
+        main_ray = !capped && !dog;
+        nx = main_ray ? NX_MAIN : NX_FINISH;
+    }
+    const bool need_ray = (is_nee && nee_ray) || main_ray;
+    if (is_nee && !nee_ray) {
+        c.st = ST_EVENT;
+        c.ev = EV_SHADOW;
+        c.nee_factor = 0.0f;
+        c.aux = 0.0f;
+    }
+    if (need_ray) {
+        c.st = ST_RAY;
+        c.d = dir;
+        c.mode = is_nee ? kTrackDelta : kTrackCollision;
+        c.nee_factor = is_nee ? factor : c.nee_factor;
+    }
+    if (nx == NX_FINISH) {
+        AdjLane a;
+        finish_path<false>(c, a, acc_sum, acc_sq, nullptr);
+    }
+}
+
 // Second half of a handler pass: one ray set-up for every lane in RAY -- a
 // shadow ray from the vertex moved by the ray epsilon (wgsl:744, 766) or a
 // main ray from the current origin (vertex already moved, or the camera).
@@ -824,7 +946,11 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
             if (!evm && !trm && !ndm) break;
             if ((evm || ndm) && ((uint32_t)__popcll(evm | ndm) >= a.event_threshold || !trm)) {
                 // handler pass: events, refill of finished lanes, ray set-ups
-                if (c.st == ST_EVENT) handle_event<ADJ, FAST>(s, msrc, c, al, A, acc_sum, acc_sq, acc_grad, wd);
+                if constexpr (FAST) {
+                    if (c.st == ST_EVENT) handle_event_fast(s, c, acc_sum, acc_sq, wd);
+                } else {
+                    if (c.st == ST_EVENT) handle_event<ADJ, FAST>(s, msrc, c, al, A, acc_sum, acc_sq, acc_grad, wd);
+                }
                 const uint64_t need = ballot(c.st == ST_NEED);
                 if (need && next < total) {
                     if (c.st == ST_NEED) {
