@@ -302,17 +302,33 @@ __device__ __forceinline__ void track_step(const DevScene &s, const MajSrc &m, L
             seg_begin<FAST>(s, m, c);
             if (c.tau < 0.0f && c.t < c.seg_end) c.tau = free_flight(rng_next(c.rng));
         }
-        if (ended) {
-            c.st = ST_EVENT;
-            c.ev = c.mode == kTrackCollision ? EV_BOUNDARY : EV_SHADOW;
-            if (c.t < c.tmax) {
-                wd = true;
-                c.aux = 0.0f;
-                c.kind = 2;  // watchdog: ends the path without contribution
-            }
-            return;
-        }
-        c.tau = -1.0f;
+        // The end of the ray, or the tentative collision and its accept test
+        // (with the albedo weight of an accepted main-ray collision), in
+        // select form: the collision's draw is taken only by lanes that have one.
+        const bool col = !ended;
+        const bool main = c.mode == kTrackCollision;
+        // (an ended lane evaluates the medium at its end point, unused)
+        const float h = lane_altitude(s, c, col ? tn : c.t);
+        const Coeffs co = lane_medium(s, c, h);
+        const float st = co.sr + co.sas + co.so + co.saa;
+        const float ratio = fminf(fmaxf(f_div(st, c.maj), 0.0f), 1.0f);
+        const float u = rng_peek(c.rng, 0);
+        c.rng.w += col ? kLcgMul : 0u;
+        const bool acc = col && u < ratio;
+        const float scat = co.sr + co.sas;
+        const bool pos = st > 0.0f && scat > 0.0f;
+        const float tpw = pos ? c.tp * fminf(fmaxf(f_div(scat, st), 0.0f), 1.0f) : 0.0f;
+        const bool dog = ended && c.t < c.tmax;  // watchdog: ends the path without contribution
+        wd = wd || dog;
+        c.t = col ? tn : c.t;
+        c.tau = col ? -1.0f : c.tau;
+        c.st = (ended || acc) ? ST_EVENT : c.st;
+        c.ev = ended ? (main ? EV_BOUNDARY : EV_SHADOW) : (acc ? (main ? EV_HIT : EV_SHADOW) : c.ev);
+        c.tp = acc && main ? tpw : c.tp;
+        c.rw = acc && main && pos ? f_div(co.sr, scat) : c.rw;
+        c.aux = dog || (acc && !main) ? 0.0f : c.aux;
+        c.kind = dog ? 2 : c.kind;
+        return;
     } else {
     const bool carried = (s.flags & kFlagCarriedTau) != 0;
     for (;;) {
