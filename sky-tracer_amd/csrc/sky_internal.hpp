@@ -85,6 +85,7 @@ struct PoolArgs {
     uint32_t event_threshold, track_steps;
     uint32_t maj_lds;  // floats per wave of the LDS majorant copy (0: read the grids from global memory)
     uint32_t shell_lds;  // doubles per wave of the LDS shell table
+    uint32_t lam_lds;    // floats per wave of the LDS per-band coefficients (FAST kernels)
 };
 
 // Inputs of the adjoint replay (sky_render_adjoint).
@@ -103,6 +104,7 @@ struct WavePlan {
     uint32_t shell_lds = 0; // doubles per wave for the shell table in LDS (0 with maj_lds 0)
     uint32_t lds_bytes = 0; // dynamic LDS per block
     bool fast = false;      // the _fast kernels (see plan_launch)
+    uint32_t lam_lds = 0;   // floats per wave for the per-band coefficients in LDS (fast kernels)
 };
 
 void plan_job(const DevScene &s, const sky_tuning &t, int num_cus, uint32_t min_items, DevJob &j);

@@ -192,6 +192,8 @@ constexpr uint32_t kMajLdsMax = 2048;
 constexpr uint32_t kLdsFixed = 2 * kSlotsPerWave + 16;
 
 typedef __attribute__((address_space(3))) const float *LdsFloatPtr;
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) const v4f *LdsFloat4Ptr;
 typedef __attribute__((address_space(1))) const float *GlobalFloatPtr;
 
 // The two sources carry their address spaces so that each load is a plain
@@ -202,6 +204,11 @@ struct MajSrc {
     GlobalFloatPtr glob;  // the job's grid in global memory
     bool use_lds;
     ShellTab shells;      // the job's shell table, LDS copy with the grid
+    // FAST kernels: the job's per-band coefficients (beta_R, ozone, beta_As,
+    // beta_Aa) and sun radiance, LDS copies read where they are used instead
+    // of being carried in registers by every lane
+    LdsFloat4Ptr lam4;
+    LdsFloatPtr lsun;
 };
 
 __device__ __forceinline__ float majorant(const MajSrc &m, uint32_t L, uint32_t band, int k)
@@ -258,9 +265,15 @@ __device__ __forceinline__ float lane_altitude(const DevScene &s, const LaneCtx 
     return altitude_bq(s, c.bf, c.qf, t);
 }
 
-__device__ __forceinline__ Coeffs lane_medium(const DevScene &s, const LaneCtx &c, float h)
+template <bool FAST>
+__device__ __forceinline__ Coeffs lane_medium(const DevScene &s, const MajSrc &m, const LaneCtx &c, float h)
 {
-    return medium4(s, c.band, c.br, c.bo, c.bas, c.baa, h);
+    if constexpr (FAST) {
+        const v4f b = m.lam4[c.band];
+        return medium4(s, c.band, b.x, b.y, b.z, b.w, h);
+    } else {
+        return medium4(s, c.band, c.br, c.bo, c.bas, c.baa, h);
+    }
 }
 
 // One tracking step.  The walk consumes shell crossings (no draw for an
@@ -309,7 +322,7 @@ __device__ __forceinline__ void track_step(const DevScene &s, const MajSrc &m, L
         const bool main = c.mode == kTrackCollision;
         // (an ended lane evaluates the medium at its end point, unused)
         const float h = lane_altitude(s, c, col ? tn : c.t);
-        const Coeffs co = lane_medium(s, c, h);
+        const Coeffs co = lane_medium<true>(s, m, c, h);
         const float st = co.sr + co.sas + co.so + co.saa;
         const float ratio = fminf(fmaxf(f_div(st, c.maj), 0.0f), 1.0f);
         const float u = rng_peek(c.rng, 0);
@@ -365,7 +378,7 @@ __device__ __forceinline__ void track_step(const DevScene &s, const MajSrc &m, L
     }
     c.t = tn;
     const float h = lane_altitude(s, c, tn);
-    Coeffs co = lane_medium(s, c, h);
+    Coeffs co = lane_medium<FAST>(s, m, c, h);
     float st = co.sr + co.sas + co.so + co.saa;
     float ratio = fminf(fmaxf(f_div(st, c.maj), 0.0f), 1.0f);
     if (!FAST && c.mode == kTrackRatio) {
@@ -658,10 +671,11 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
 // rays, no adjoint): the same steps with every branch that has no side effect
 // turned into selects, so a pass costs one evaluation of each piece instead of
 // the exec-mask bookkeeping of nested branches.  Same draws, same arithmetic.
-__device__ __forceinline__ void handle_event_fast(const DevScene &s, LaneCtx &c, float *acc_sum, float *acc_sq,
-                                                  bool &wd)
+__device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSrc &m, LaneCtx &c, float *acc_sum,
+                                                  float *acc_sq, bool &wd)
 {
     const uint32_t n_light = s.dls;
+    const float lsun = m.lsun[c.band];
 
     // 1. the event
     const bool bnd = c.ev == EV_BOUNDARY, hit = c.ev == EV_HIT, shd = !bnd && !hit;
@@ -677,7 +691,7 @@ __device__ __forceinline__ void handle_event_fast(const DevScene &s, LaneCtx &c,
     }
     // exit: the sun disk is seen only by the camera ray (wgsl:745-749)
     const bool sun_seen = bnd && c.kind == 1 && c.depth == 0 && in_sun_cone(s, c.d);
-    c.L = sun_seen ? c.L + c.tp * c.lsun : c.L;
+    c.L = sun_seen ? c.L + c.tp * lsun : c.L;
     wd = wd || (bnd && c.kind == 3);
     const bool scatter_vertex = hit && c.tp > 0.0f;
     const bool nee_start = at_ground || scatter_vertex;
@@ -726,7 +740,7 @@ __device__ __forceinline__ void handle_event_fast(const DevScene &s, LaneCtx &c,
     const float cs = fmaxf(dot3(gn, dir), 0.0f);
     const float mu_v = dot3(c.vin, dir);
     const float ph = c.rw * phase_rayleigh(mu_v) + (1.0f - c.rw) * phase_hg(s.g, mu_v);
-    const float factor = wdiv<true>(c.vtype == 1 ? s.albedo * kInvPi * c.lsun * cs : c.lsun * ph, s.cone_pdf);
+    const float factor = wdiv<true>(c.vtype == 1 ? s.albedo * kInvPi * lsun * cs : lsun * ph, s.cone_pdf);
     const bool nee_ray = c.vtype == 1 ? cs > 0.0f : true;
     // CONT: the NEE average, the bounce weight (1 for lobe sampling, albedo
     // for the cosine lobe) and the roulette (wgsl:691-701, 751-770)
@@ -810,7 +824,7 @@ struct ItemInfo {
 
 // Path start (main, sky_trace.wgsl:799-807): triple j of the item, ordered
 // (sample, pixel, band) with the band fastest.
-template <bool ADJ>
+template <bool ADJ, bool FAST>
 __device__ __forceinline__ void start_path(const DevScene &s, LaneCtx &c, AdjLane &a, const AdjArgs &A,
                                            uint32_t j, const ItemInfo &it)
 {
@@ -832,12 +846,14 @@ __device__ __forceinline__ void start_path(const DevScene &s, LaneCtx &c, AdjLan
         y += dy;
     }
     const uint32_t pixel = it.p0 + pl;
-    const DevLambda l = s.lam[c.band];
-    c.br = l.br;
-    c.bo = l.bo;
-    c.bas = l.bas;
-    c.baa = l.baa;
-    c.lsun = l.lsun;
+    if constexpr (!FAST) {
+        const DevLambda l = s.lam[c.band];
+        c.br = l.br;
+        c.bo = l.bo;
+        c.bas = l.bas;
+        c.baa = l.baa;
+        c.lsun = l.lsun;
+    }
     c.rng = rng_make(pixel, it.s0 + si, c.band, s);
     float j1 = rng_peek(c.rng, 0);
     float j2 = rng_peek(c.rng, 1);
@@ -864,6 +880,7 @@ __device__ __forceinline__ void start_path(const DevScene &s, LaneCtx &c, AdjLan
     }
     if constexpr (ADJ) {
         const float *g = A.dl_dxyz + 3 * (size_t)pixel;
+        const DevLambda l = s.lam[c.band];
         a.w = (g[0] * l.wx + g[1] * l.wy + g[2] * l.wz) * A.inv_spp;
         for (int k = 0; k < kNP; ++k) {
             a.S[k] = 0.0f;
@@ -894,12 +911,15 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
     uint32_t loaded = 0xFFFFFFFFu;  // job whose majorants are in the wave's LDS copy
     const uint32_t wid = threadIdx.x >> 6;
     const uint32_t lane = lane_id();
-    float *wave_lds = dyn_lds + wid * (kLdsFixed + a.maj_lds + 2 * a.shell_lds);
+    float *wave_lds = dyn_lds + wid * (kLdsFixed + a.maj_lds + 2 * a.shell_lds + a.lam_lds);
     float *acc_sum = wave_lds;
     float *acc_sq = wave_lds + kSlotsPerWave;
     float *acc_grad = wave_lds + 2 * kSlotsPerWave;
     float *lds_maj = wave_lds + kLdsFixed;
     double *lds_shell = (double *)(lds_maj + a.maj_lds);
+    // FAST: [band] 4-float coefficients, then [band] sun radiance (kMaxLambda bands each)
+    v4f *lds_lam4 = (v4f *)(lds_maj + a.maj_lds + 2 * a.shell_lds);
+    float *lds_lsun = (float *)(lds_lam4 + (uint32_t)kMaxLambda);
     bool wd = false;
     uint32_t *diag = nullptr;
     LaneCtx c;
@@ -941,10 +961,19 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
         msrc.shells.lds = (LdsDoublePtr)lds_shell;
         msrc.shells.glob = (GlobalDoublePtr)s.shell_d;
         msrc.shells.use_lds = msrc.use_lds;
+        msrc.lam4 = (LdsFloat4Ptr)lds_lam4;
+        msrc.lsun = (LdsFloatPtr)lds_lsun;
         if (msrc.use_lds) {
             if (loaded != jid) {
                 for (uint32_t i = lane; i < nmaj; i += 64) lds_maj[i] = s.maj[i];
                 for (uint32_t i = lane; i <= s.L; i += 64) lds_shell[i] = s.shell_d[i];
+                if constexpr (FAST) {
+                    for (uint32_t i = lane; i < s.nlam; i += 64) {
+                        const DevLambda l = s.lam[i];
+                        lds_lam4[i] = v4f{l.br, l.bo, l.bas, l.baa};
+                        lds_lsun[i] = l.lsun;
+                    }
+                }
                 loaded = jid;
             }
         }
@@ -963,7 +992,7 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
             if ((evm || ndm) && ((uint32_t)__popcll(evm | ndm) >= a.event_threshold || !trm)) {
                 // handler pass: events, refill of finished lanes, ray set-ups
                 if constexpr (FAST) {
-                    if (c.st == ST_EVENT) handle_event_fast(s, c, acc_sum, acc_sq, wd);
+                    if (c.st == ST_EVENT) handle_event_fast(s, msrc, c, acc_sum, acc_sq, wd);
                 } else {
                     if (c.st == ST_EVENT) handle_event<ADJ, FAST>(s, msrc, c, al, A, acc_sum, acc_sq, acc_grad, wd);
                 }
@@ -972,7 +1001,7 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
                     if (c.st == ST_NEED) {
                         uint32_t j = next + mbcnt(need);
                         if (j < total)
-                            start_path<ADJ>(s, c, al, A, j, it);
+                            start_path<ADJ, FAST>(s, c, al, A, j, it);
                         else
                             c.st = ST_IDLE;
                     }
@@ -1112,7 +1141,8 @@ WavePlan plan_launch(const sky_tuning &t, int num_cus, uint32_t total_items, uin
     const uint32_t fast_flags = kFlagCarriedTau | kFlagPhaseIS;
     p.fast = p.maj_lds != 0 && max_layers <= kFastMaxShells && (flags_all & (fast_flags | kFlagRatio)) == fast_flags;
     p.shell_lds = p.maj_lds ? ((max_layers + 1u + 7u) & ~7u) : 0u;
-    p.lds_bytes = kWavesPerBlock * (kLdsFixed + p.maj_lds + 2 * p.shell_lds) * (uint32_t)sizeof(float);
+    p.lam_lds = p.fast ? 5u * (uint32_t)kMaxLambda : 0u;
+    p.lds_bytes = kWavesPerBlock * (kLdsFixed + p.maj_lds + 2 * p.shell_lds + p.lam_lds) * (uint32_t)sizeof(float);
     uint32_t wpc = t.waves_per_cu ? t.waves_per_cu : 16u;
     uint64_t waves = (uint64_t)(num_cus > 0 ? num_cus : 256) * wpc;
     uint64_t w = waves < total_items ? waves : total_items;
@@ -1147,6 +1177,7 @@ hipError_t launch_pool(const DevJob *d_jobs, uint32_t n_jobs, uint32_t max_npx, 
     a.track_steps = p.track_steps;
     a.maj_lds = p.maj_lds;
     a.shell_lds = p.shell_lds;
+    a.lam_lds = p.lam_lds;
     typedef void (*PoolKernel)(PoolArgs);
     PoolKernel k;
     switch (p.occupancy) {
@@ -1177,6 +1208,7 @@ hipError_t launch_adjoint(const DevJob *d_jobs, const WavePlan &p, uint32_t *met
     a.track_steps = p.track_steps;
     a.maj_lds = p.maj_lds;
     a.shell_lds = p.shell_lds;
+    a.lam_lds = p.lam_lds;
     hipLaunchKernelGGL(k_wavepool_adjoint, dim3(p.grid), dim3(p.block), p.lds_bytes, stream, a, A);
     return hipGetLastError();
 }
