@@ -44,6 +44,18 @@ TRAFFIC_BYTES = {"c2": (2.0 * 1853.5 + 278208.1) * 1024}
 SCAN_TRAFFIC_BYTES = {6: (2.0 * 5189454.0 + 70256.1) * 1024}
 
 
+def pool_kernel(mode, params, n_lambda):
+    """Name of the integrator kernel a render launches: the library picks the
+    _fast variants for carried optical depth + lobe sampling + delta-tracked
+    shadow rays with at most 8 majorant shells whose grid fits its LDS copy
+    (n_lambda x shells <= 2048), and the register-budget variant from the
+    tuning's occupancy."""
+    layers = params["majorant_layers"]
+    fast = mode == "carried_is" and layers <= 8 and n_lambda * layers <= 2048
+    occ = params.get("occupancy", 0)
+    return "k_wavepool" + ("_fast" if fast else "") + (f"_o{occ}" if occ else "")
+
+
 def make_scene(w):
     import pysky
 
@@ -571,7 +583,8 @@ def main():
                         "frac_nonpacked": round(achieved / (PEAK_FP32_TFLOPS / 2), 5),
                         "rng_int_tops": round(fm["configs"][key]["rng_int_ops_per_sample"] * samples
                                               / avg_kernel_s / 1e12, 3),
-                        "kernel": "k_wavepool", "kernel_ms_avg": round(avg_kernel_s * 1e3, 2)}
+                        "kernel": pool_kernel(args.mode, params, w["n_lambda"]),
+                        "kernel_ms_avg": round(avg_kernel_s * 1e3, 2)}
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.workload, w, args.mode, args.layers)
