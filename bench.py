@@ -38,7 +38,7 @@ PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 (packed vector FMA); unpacked vector iss
 # a 256 MiB copy): profiles/k_wavepool_c2_1024x512x64_counters.md.  The launch writes one
 # [pixel][band] float2 partial per pixel group and reads only the scene tables, so the figure
 # does not depend on spp at the tuned work split.
-TRAFFIC_BYTES = {"c2": (2.0 * 1853.5 + 278208.1) * 1024}
+TRAFFIC_BYTES = {"c2": (2.0 * 219.3 + 276672.1) * 1024}
 # Same for the combo scan of all 6-band combos x 65536 samples:
 # profiles/k_scan_6band_combos_x_65536_samples_counters.md.
 SCAN_TRAFFIC_BYTES = {6: (2.0 * 5189454.0 + 70256.1) * 1024}
