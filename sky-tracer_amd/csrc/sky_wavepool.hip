@@ -281,68 +281,11 @@ __device__ __forceinline__ Coeffs lane_medium(const DevScene &s, const MajSrc &m
 // carried optical depth, a crossing uses up part of the current draw) until
 // a tentative collision lies inside the current segment or the ray ends.
 // The collision is then evaluated and accepted or rejected in the same step.
-// FAST (carried optical depth, lobe sampling, delta-tracked shadow rays,
-// tables in LDS) draws the step's free flight before the walk: the first
-// draw of a step is made in its first non-empty segment either way, so the
-// random sequence is unchanged, and the walk loop is left with crossings only.
-template <bool ADJ, bool FAST>
+template <bool ADJ>
 __device__ __forceinline__ void track_step(const DevScene &s, const MajSrc &m, LaneCtx &c, AdjLane &a,
                                            const AdjArgs &A, bool &wd)
 {
     float tn = 0.0f;
-    if constexpr (FAST) {
-        // Each trip either finds the collision inside the current segment or
-        // crosses into the next one, with selects in place of branches and
-        // one exit per outcome; same arithmetic and draw order as below.
-        if (c.tau < 0.0f && c.t < c.seg_end) c.tau = free_flight(rng_next(c.rng));
-        bool ended = false;
-        for (;;) {
-            const bool nonempty = c.t < c.seg_end;
-            const float seg_tau = c.maj * (c.seg_end - c.t);
-            const bool cross = !nonempty || c.tau >= seg_tau;
-            tn = c.t + f_div(c.tau, c.maj);
-            if (!cross && tn < c.seg_end) break;
-            // a draw that lands at or past the segment end by rounding is spent
-            c.tau = cross ? (nonempty ? c.tau - seg_tau : c.tau) : -1.0f;
-            c.t = c.seg_end;
-            if ((c.t >= c.tmax) | (c.guard == 0u)) {
-                ended = true;
-                break;
-            }
-            c.guard--;
-            c.k = c.kn;
-            c.out = c.on;
-            seg_begin<FAST>(s, m, c);
-            if (c.tau < 0.0f && c.t < c.seg_end) c.tau = free_flight(rng_next(c.rng));
-        }
-        // The end of the ray, or the tentative collision and its accept test
-        // (with the albedo weight of an accepted main-ray collision), in
-        // select form: the collision's draw is taken only by lanes that have one.
-        const bool col = !ended;
-        const bool main = c.mode == kTrackCollision;
-        // (an ended lane evaluates the medium at its end point, unused)
-        const float h = lane_altitude(s, c, col ? tn : c.t);
-        const Coeffs co = lane_medium<true>(s, m, c, h);
-        const float st = co.sr + co.sas + co.so + co.saa;
-        const float ratio = fminf(fmaxf(f_div(st, c.maj), 0.0f), 1.0f);
-        const float u = rng_peek(c.rng, 0);
-        c.rng.w += col ? kLcgMul : 0u;
-        const bool acc = col && u < ratio;
-        const float scat = co.sr + co.sas;
-        const bool pos = st > 0.0f && scat > 0.0f;
-        const float tpw = pos ? c.tp * fminf(fmaxf(f_div(scat, st), 0.0f), 1.0f) : 0.0f;
-        const bool dog = ended && c.t < c.tmax;  // watchdog: ends the path without contribution
-        wd = wd || dog;
-        c.t = col ? tn : c.t;
-        c.tau = col ? -1.0f : c.tau;
-        c.st = (ended || acc) ? ST_EVENT : c.st;
-        c.ev = ended ? (main ? EV_BOUNDARY : EV_SHADOW) : (acc ? (main ? EV_HIT : EV_SHADOW) : c.ev);
-        c.tp = acc && main ? tpw : c.tp;
-        c.rw = acc && main && pos ? f_div(co.sr, scat) : c.rw;
-        c.aux = dog || (acc && !main) ? 0.0f : c.aux;
-        c.kind = dog ? 2 : c.kind;
-        return;
-    } else {
     const bool carried = (s.flags & kFlagCarriedTau) != 0;
     for (;;) {
         if (c.t < c.seg_end) {
@@ -373,15 +316,14 @@ __device__ __forceinline__ void track_step(const DevScene &s, const MajSrc &m, L
         c.guard++;
         c.k = c.kn;
         c.out = c.on;
-        seg_begin<FAST>(s, m, c);
-    }
+        seg_begin<false>(s, m, c);
     }
     c.t = tn;
     const float h = lane_altitude(s, c, tn);
-    Coeffs co = lane_medium<FAST>(s, m, c, h);
+    Coeffs co = lane_medium<false>(s, m, c, h);
     float st = co.sr + co.sas + co.so + co.saa;
     float ratio = fminf(fmaxf(f_div(st, c.maj), 0.0f), 1.0f);
-    if (!FAST && c.mode == kTrackRatio) {
+    if (c.mode == kTrackRatio) {
         c.aux *= 1.0f - ratio;
         if constexpr (ADJ) {
             float f = -1.0f / fmaxf(c.maj - st, 1e-30f);
@@ -423,6 +365,66 @@ __device__ __forceinline__ void track_step(const DevScene &s, const MajSrc &m, L
     }
 }
 
+// The tracking step of the FAST kernels (carried optical depth, lobe
+// sampling, delta-tracked shadow rays, tables in LDS): the same walk, draws
+// and arithmetic as track_step, written for SIMT.  The step's free flight is
+// drawn before the walk (the first draw of a step is made in its first
+// non-empty segment either way, so the random sequence is unchanged); each
+// trip of the walk either finds the collision inside the current segment or
+// crosses into the next one, with selects in place of branches and one exit
+// per outcome; the collision test is in select form too.
+__device__ __forceinline__ void track_step_fast(const DevScene &s, const MajSrc &m, LaneCtx &c, bool &wd)
+{
+    float tn = 0.0f;
+    if (c.tau < 0.0f && c.t < c.seg_end) c.tau = free_flight(rng_next(c.rng));
+    bool ended = false;
+    for (;;) {
+        const bool nonempty = c.t < c.seg_end;
+        const float seg_tau = c.maj * (c.seg_end - c.t);
+        const bool cross = !nonempty || c.tau >= seg_tau;
+        tn = c.t + f_div(c.tau, c.maj);
+        if (!cross && tn < c.seg_end) break;
+        // a draw that lands at or past the segment end by rounding is spent
+        c.tau = cross ? (nonempty ? c.tau - seg_tau : c.tau) : -1.0f;
+        c.t = c.seg_end;
+        if ((c.t >= c.tmax) | (c.guard == 0u)) {
+            ended = true;
+            break;
+        }
+        c.guard--;
+        c.k = c.kn;
+        c.out = c.on;
+        seg_begin<true>(s, m, c);
+        if (c.tau < 0.0f && c.t < c.seg_end) c.tau = free_flight(rng_next(c.rng));
+    }
+    // The end of the ray, or the tentative collision and its accept test
+    // (with the albedo weight of an accepted main-ray collision), in
+    // select form: the collision's draw is taken only by lanes that have one.
+    const bool col = !ended;
+    const bool main = c.mode == kTrackCollision;
+    // (an ended lane evaluates the medium at its end point, unused)
+    const float h = lane_altitude(s, c, col ? tn : c.t);
+    const Coeffs co = lane_medium<true>(s, m, c, h);
+    const float st = co.sr + co.sas + co.so + co.saa;
+    const float ratio = fminf(fmaxf(f_div(st, c.maj), 0.0f), 1.0f);
+    const float u = rng_peek(c.rng, 0);
+    c.rng.w += col ? kLcgMul : 0u;
+    const bool acc = col && u < ratio;
+    const float scat = co.sr + co.sas;
+    const bool pos = st > 0.0f && scat > 0.0f;
+    const float tpw = pos ? c.tp * fminf(fmaxf(f_div(scat, st), 0.0f), 1.0f) : 0.0f;
+    const bool dog = ended && c.t < c.tmax;  // watchdog: ends the path without contribution
+    wd = wd || dog;
+    c.t = col ? tn : c.t;
+    c.tau = col ? -1.0f : c.tau;
+    c.st = (ended || acc) ? ST_EVENT : c.st;
+    c.ev = ended ? (main ? EV_BOUNDARY : EV_SHADOW) : (acc ? (main ? EV_HIT : EV_SHADOW) : c.ev);
+    c.tp = acc && main ? tpw : c.tp;
+    c.rw = acc && main && pos ? f_div(co.sr, scat) : c.rw;
+    c.aux = dog || (acc && !main) ? 0.0f : c.aux;
+    c.kind = dog ? 2 : c.kind;
+}
+
 // Vertex at distance t along the main ray from (px, py, pz), FP64.
 __device__ __forceinline__ void advance_vertex(LaneCtx &c, float t)
 {
@@ -435,16 +437,7 @@ __device__ __forceinline__ void advance_vertex(LaneCtx &c, float t)
 
 enum Next : int { NX_NEE = 1, NX_CONT = 2, NX_MAIN = 3, NX_FINISH = 4 };
 
-// Weight divisions of the event handler: the FAST kernels multiply by the
-// hardware reciprocal (within 1 ulp of the quotient), the others divide.
-template <bool FAST>
-__device__ __forceinline__ float wdiv(float a, float b)
-{
-    if constexpr (FAST)
-        return f_div(a, b);
-    else
-        return a / b;
-}
+
 
 // Path end: the path's radiance and its square into the wave's LDS sums.
 template <bool ADJ>
@@ -469,13 +462,13 @@ __device__ __forceinline__ void finish_path(LaneCtx &c, AdjLane &a, float *acc_s
 // the ray set-up: it leaves the lane in RAY (c.d, c.mode and, for a shadow
 // ray, c.nee_factor set), EVENT (a zero sun sample, for the next pass) or
 // NEED (path finished and accumulated).
-template <bool ADJ, bool FAST>
+template <bool ADJ>
 __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m, LaneCtx &c, AdjLane &a,
                                              const AdjArgs &A, float *acc_sum, float *acc_sq, float *acc_grad,
                                              bool &wd)
 {
     const uint32_t n_light = s.dls;
-    const bool phase_is = FAST || (s.flags & kFlagPhaseIS) != 0;
+    const bool phase_is = (s.flags & kFlagPhaseIS) != 0;
     int nx = NX_FINISH;
 
     // 1. the event; a ground hit or an accepted collision moves the vertex
@@ -574,7 +567,7 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
     if (nx == NX_NEE) {
         if (c.vtype == 1) {
             float cs = fmaxf(dot3(nrm, dir), 0.0f);
-            factor = wdiv<FAST>(s.albedo * kInvPi * c.lsun * cs, s.cone_pdf);
+            factor = s.albedo * kInvPi * c.lsun * cs / s.cone_pdf;
             need_ray = cs > 0.0f;
             if constexpr (ADJ) {
                 for (int k = 0; k < kNP; ++k) a.SS[k] = a.S[k];
@@ -583,14 +576,14 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
         } else {
             float mu = dot3(c.vin, dir);
             float ph = c.rw * phase_rayleigh(mu) + (1.0f - c.rw) * phase_hg(s.g, mu);
-            factor = wdiv<FAST>(c.lsun * ph, s.cone_pdf);
+            factor = c.lsun * ph / s.cone_pdf;
             need_ray = true;
             if constexpr (ADJ) {
                 for (int k = 0; k < kNP; ++k) a.SS[k] = a.S[k];
                 add_dlog_phase(s, A, a, c.band, mu, c.bo, a.SS);
             }
         }
-        mode = (!FAST && (s.flags & kFlagRatio)) ? kTrackRatio : kTrackDelta;
+        mode = (s.flags & kFlagRatio) ? kTrackRatio : kTrackDelta;
         if (!need_ray) {
             // ground facing away from the sun: a zero sample
             c.st = ST_EVENT;
@@ -599,7 +592,7 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
             c.aux = 0.0f;
         }
     } else if (nx == NX_CONT) {
-        c.L = c.L + c.tp * wdiv<FAST>(c.nee_sum, (float)n_light);
+        c.L = c.L + c.tp * (c.nee_sum / (float)n_light);
         float w;
         bool ok = true;
         if (c.vtype == 0) {
@@ -626,7 +619,7 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
                 if (urr > q)
                     live = false;
                 else
-                    c.tp = wdiv<FAST>(c.tp, q);
+                    c.tp = c.tp / q;
             }
             if (live) {
                 if (c.L < 0.0f || c.L != c.L || c.tp < 0.0f || c.tp != c.tp) {
@@ -670,7 +663,9 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
 // handle_event for the FAST kernels (lobe sampling, delta-tracked shadow
 // rays, no adjoint): the same steps with every branch that has no side effect
 // turned into selects, so a pass costs one evaluation of each piece instead of
-// the exec-mask bookkeeping of nested branches.  Same draws, same arithmetic.
+// the exec-mask bookkeeping of nested branches.  Same draws and arithmetic,
+// except that the four weight divisions multiply by the hardware reciprocal
+// (within 1 ulp of the quotient).
 __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSrc &m, LaneCtx &c, float *acc_sum,
                                                   float *acc_sq, bool &wd)
 {
@@ -740,16 +735,16 @@ __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSr
     const float cs = fmaxf(dot3(gn, dir), 0.0f);
     const float mu_v = dot3(c.vin, dir);
     const float ph = c.rw * phase_rayleigh(mu_v) + (1.0f - c.rw) * phase_hg(s.g, mu_v);
-    const float factor = wdiv<true>(c.vtype == 1 ? s.albedo * kInvPi * lsun * cs : lsun * ph, s.cone_pdf);
+    const float factor = f_div(c.vtype == 1 ? s.albedo * kInvPi * lsun * cs : lsun * ph, s.cone_pdf);
     const bool nee_ray = c.vtype == 1 ? cs > 0.0f : true;
     // CONT: the NEE average, the bounce weight (1 for lobe sampling, albedo
     // for the cosine lobe) and the roulette (wgsl:691-701, 751-770)
-    const float Lc = c.L + c.tp * wdiv<true>(c.nee_sum, (float)n_light);
+    const float Lc = c.L + c.tp * f_div(c.nee_sum, (float)n_light);
     const bool ok = scat_v || cs > 0.0f;
     const float tp1 = c.tp * (scat_v ? 1.0f : s.albedo);
     const float q = fminf(fmaxf(tp1, 0.05f), 0.95f);
     const bool killed = rr && (scat_v ? u3 : u2) > q;
-    const float tp2 = rr ? wdiv<true>(tp1, q) : tp1;
+    const float tp2 = rr ? f_div(tp1, q) : tp1;
     const bool live = ok && !killed;
     const bool bad = Lc < 0.0f || Lc != Lc || tp2 < 0.0f || tp2 != tp2;
     const bool bounce = is_cont && live && !bad;
@@ -994,7 +989,7 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
                 if constexpr (FAST) {
                     if (c.st == ST_EVENT) handle_event_fast(s, msrc, c, acc_sum, acc_sq, wd);
                 } else {
-                    if (c.st == ST_EVENT) handle_event<ADJ, FAST>(s, msrc, c, al, A, acc_sum, acc_sq, acc_grad, wd);
+                    if (c.st == ST_EVENT) handle_event<ADJ>(s, msrc, c, al, A, acc_sum, acc_sq, acc_grad, wd);
                 }
                 const uint64_t need = ballot(c.st == ST_NEED);
                 if (need && next < total) {
@@ -1011,7 +1006,12 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
             }
             for (uint32_t i = 0; i < a.track_steps; ++i) {
                 if (!ballot(c.st == ST_TRACK)) break;
-                if (c.st == ST_TRACK) track_step<ADJ, FAST>(s, msrc, c, al, A, wd);
+                if (c.st == ST_TRACK) {
+                    if constexpr (FAST)
+                        track_step_fast(s, msrc, c, wd);
+                    else
+                        track_step<ADJ>(s, msrc, c, al, A, wd);
+                }
             }
         }
         __builtin_amdgcn_wave_barrier();
