@@ -248,11 +248,12 @@ def bench_adjoint(args, ctx):
     # loss = mean(log XYZ) over the image, so every pixel weighs by its relative
     # change (an XYZ^2 loss would be the sun disk's transmittance alone);
     # its cotangent 1 / (n XYZ) comes from an independent 4x-spp render
-    render_device(scene, dataclasses.replace(base, spp=4 * spp, seed=99), xyz=xyz, context=ctx)
+    render_device(scene, dataclasses.replace(base, spp=4 * spp, seed=pysky.independent_seed(99)), xyz=xyz,
+                  context=ctx)
     cot = (1.0 / (xyz.numel() * torch.clamp(xyz, min=1e-3 * float(xyz.mean())))).contiguous()
     grads, f_ms, a_ms = [], [], []
     for i in range(max(args.warmup, 1) + 8):
-        cfg = dataclasses.replace(base, seed=100 + i)
+        cfg = dataclasses.replace(base, seed=pysky.independent_seed(100 + i))
         st = render_device(scene, cfg, xyz=xyz, context=ctx)
         g, ast = render_adjoint_device(scene, cfg, cot, context=ctx)
         if i >= max(args.warmup, 1):

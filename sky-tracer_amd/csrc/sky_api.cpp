@@ -444,7 +444,7 @@ sky_status adjoint_impl(sky_context *ctx, const sky_planet *planet, const sky_at
     j.done = (uint32_t *)ctx->meta.p + 4;
     j.t_done = (unsigned long long *)((uint32_t *)ctx->meta.p + 6);
     HIP_TRY(hipMemcpyAsync(ctx->jobs.p, &j, sizeof(j), hipMemcpyHostToDevice, stream), "upload job", SKY_ERR_COPY);
-    sky::WavePlan plan = sky::plan_launch(ctx->tuning, ctx->num_cus, j.n_items, s.nlam * s.L, s.L, sky::kFlagMixed);
+    sky::WavePlan plan = sky::plan_launch(ctx->tuning, ctx->num_cus, j.n_items, s.nlam * s.L, s.L, s.flags);
     sky::AdjArgs A;
     A.dl_dxyz = d_dl;
     A.lam_units = (const float4 *)ctx->adj_units.p;

@@ -373,7 +373,9 @@ __device__ __forceinline__ void track_step(const DevScene &s, const MajSrc &m, L
 // trip of the walk either finds the collision inside the current segment or
 // crosses into the next one, with selects in place of branches and one exit
 // per outcome; the collision test is in select form too.
-__device__ __forceinline__ void track_step_fast(const DevScene &s, const MajSrc &m, LaneCtx &c, bool &wd)
+template <bool ADJ>
+__device__ __forceinline__ void track_step_fast(const DevScene &s, const MajSrc &m, LaneCtx &c, AdjLane &a,
+                                                const AdjArgs &A, bool &wd)
 {
     float tn = 0.0f;
     if (c.tau < 0.0f && c.t < c.seg_end) c.tau = free_flight(rng_next(c.rng));
@@ -423,6 +425,28 @@ __device__ __forceinline__ void track_step_fast(const DevScene &s, const MajSrc 
     c.rw = acc && main && pos ? f_div(co.sr, scat) : c.rw;
     c.aux = dog || (acc && !main) ? 0.0f : c.aux;
     c.kind = dog ? 2 : c.kind;
+    if constexpr (ADJ) {
+        // accepted main-ray collision: d log(sigma_s / mu) into S; null
+        // collision: d log(1 - sigma_t / mu) into S (main ray) or SS (shadow)
+        const DSig d = dsig(s, A, c.band, h, m.lam4[c.band].y, co);
+        const bool rej = col && !acc;
+        const float fs = acc && main && scat > 0.0f ? 1.0f / scat : 0.0f;
+        const float fr = rej ? -1.0f / fmaxf(c.maj - st, 1e-30f) : 0.0f;
+        const float fm = main ? fr : 0.0f, fh = main ? 0.0f : fr;
+        a.S[0] += (fs + fm) * d.r0;
+        a.S[1] += (fs + fm) * d.r1;
+        a.S[2] += fm * d.o2;
+        a.S[3] += fm * d.o3;
+        a.S[4] += (fs + fm) * d.as4;
+        a.S[5] += fm * d.aa5;
+        a.S[6] += fs * d.as6 + fm * (d.as6 + d.aa6);
+        a.S[8] += fs * d.as8 + fm * (d.as8 + d.aa8);
+        add_dsigma_t(a.SS, d, fh);
+        const bool vert = acc && main;
+        a.vsr = vert ? co.sr : a.vsr;
+        a.vsas = vert ? co.sas : a.vsas;
+        a.vh = vert ? h : a.vh;
+    }
 }
 
 // Vertex at distance t along the main ray from (px, py, pz), FP64.
@@ -666,8 +690,10 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
 // the exec-mask bookkeeping of nested branches.  Same draws and arithmetic,
 // except that the four weight divisions multiply by the hardware reciprocal
 // (within 1 ulp of the quotient).
-__device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSrc &m, LaneCtx &c, float *acc_sum,
-                                                  float *acc_sq, bool &wd)
+template <bool ADJ>
+__device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSrc &m, LaneCtx &c, AdjLane &a,
+                                                  const AdjArgs &A, float *acc_sum, float *acc_sq, float *acc_grad,
+                                                  bool &wd)
 {
     const uint32_t n_light = s.dls;
     const float lsun = m.lsun[c.band];
@@ -686,6 +712,15 @@ __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSr
     }
     // exit: the sun disk is seen only by the camera ray (wgsl:745-749)
     const bool sun_seen = bnd && c.kind == 1 && c.depth == 0 && in_sun_cone(s, c.d);
+    if constexpr (ADJ) {
+        // the sun disk's contribution with the path's score, a sun sample's
+        // with the sample's score
+        const float Csun = c.tp * lsun;
+        const float Csh = c.tp * c.nee_factor * c.aux / (float)n_light;
+        const bool gh = shd && Csh != 0.0f;
+        for (int k = 0; k < kNP; ++k)
+            a.G[k] += (sun_seen ? a.w * Csun * a.S[k] : 0.0f) + (gh ? a.w * Csh * a.SS[k] : 0.0f);
+    }
     c.L = sun_seen ? c.L + c.tp * lsun : c.L;
     wd = wd || (bnd && c.kind == 3);
     const bool scatter_vertex = hit && c.tp > 0.0f;
@@ -748,6 +783,21 @@ __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSr
     const bool live = ok && !killed;
     const bool bad = Lc < 0.0f || Lc != Lc || tp2 < 0.0f || tp2 != tp2;
     const bool bounce = is_cont && live && !bad;
+    if constexpr (ADJ) {
+        // NEE: the sun sample's score starts from the path's plus its phase
+        // (scattering vertex) or albedo (ground) term; CONT: the bounce adds
+        // the same terms to the path's score
+        float D[kNP];
+        for (int k = 0; k < kNP; ++k) D[k] = 0.0f;
+        add_dlog_phase(s, A, a, c.band, mu_v, m.lam4[c.band].y, D);
+        const float alb = s.albedo > 0.0f ? 1.0f / s.albedo : 0.0f;
+        for (int k = 0; k < kNP; ++k) {
+            const float t = scat_v ? D[k] : (k == 9 ? alb : 0.0f);
+            a.SS[k] = is_nee ? a.S[k] + t : a.SS[k];
+            a.S[k] = is_cont ? a.S[k] + t : a.S[k];
+            a.G[k] = is_cont && live && bad ? 0.0f : a.G[k];
+        }
+    }
     if (is_cont) {
         c.L = live && bad ? 0.0f : Lc;
         c.tp = ok ? (killed ? tp1 : tp2) : c.tp;
@@ -779,10 +829,7 @@ __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSr
         c.mode = is_nee ? kTrackDelta : kTrackCollision;
         c.nee_factor = is_nee ? factor : c.nee_factor;
     }
-    if (nx == NX_FINISH) {
-        AdjLane a;
-        finish_path<false>(c, a, acc_sum, acc_sq, nullptr);
-    }
+    if (nx == NX_FINISH) finish_path<ADJ>(c, a, acc_sum, acc_sq, acc_grad);
 }
 
 // Second half of a handler pass: one ray set-up for every lane in RAY -- a
@@ -987,7 +1034,7 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
             if ((evm || ndm) && ((uint32_t)__popcll(evm | ndm) >= a.event_threshold || !trm)) {
                 // handler pass: events, refill of finished lanes, ray set-ups
                 if constexpr (FAST) {
-                    if (c.st == ST_EVENT) handle_event_fast(s, msrc, c, acc_sum, acc_sq, wd);
+                    if (c.st == ST_EVENT) handle_event_fast<ADJ>(s, msrc, c, al, A, acc_sum, acc_sq, acc_grad, wd);
                 } else {
                     if (c.st == ST_EVENT) handle_event<ADJ>(s, msrc, c, al, A, acc_sum, acc_sq, acc_grad, wd);
                 }
@@ -1008,7 +1055,7 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
                 if (!ballot(c.st == ST_TRACK)) break;
                 if (c.st == ST_TRACK) {
                     if constexpr (FAST)
-                        track_step_fast(s, msrc, c, wd);
+                        track_step_fast<ADJ>(s, msrc, c, al, A, wd);
                     else
                         track_step<ADJ>(s, msrc, c, al, A, wd);
                 }
@@ -1059,6 +1106,7 @@ SKY_WAVEPOOL_OCC(8)
 
 // Path-replay adjoint: the forward state machine plus the score bookkeeping.
 __global__ __launch_bounds__(256) void k_wavepool_adjoint(PoolArgs a, AdjArgs A) { wavepool_body<true, false>(a, A); }
+__global__ __launch_bounds__(256) void k_wavepool_adjoint_fast(PoolArgs a, AdjArgs A) { wavepool_body<true, true>(a, A); }
 
 // Per job (blockIdx.y): reduces the chunk partials in chunk order,
 // normalises by 1/spp and clamps (integrator.rs:307-310), and forms
@@ -1209,7 +1257,8 @@ hipError_t launch_adjoint(const DevJob *d_jobs, const WavePlan &p, uint32_t *met
     a.maj_lds = p.maj_lds;
     a.shell_lds = p.shell_lds;
     a.lam_lds = p.lam_lds;
-    hipLaunchKernelGGL(k_wavepool_adjoint, dim3(p.grid), dim3(p.block), p.lds_bytes, stream, a, A);
+    hipLaunchKernelGGL(p.fast ? k_wavepool_adjoint_fast : k_wavepool_adjoint, dim3(p.grid), dim3(p.block),
+                       p.lds_bytes, stream, a, A);
     return hipGetLastError();
 }
 

@@ -23,6 +23,7 @@ from .api import (  # noqa: F401
     default_context,
     load_scene,
     majorant_grid,
+    independent_seed,
     midpoint_grid,
     optical_depth,
     render,

@@ -189,6 +189,18 @@ PARAM_NAMES = (
 )
 
 
+def independent_seed(k: int) -> int:
+    """Seed of the k-th of a set of statistically independent renders.
+
+    A path's random numbers are keyed by (pixel ^ seed_lo, sample ^ seed_hi,
+    band, dimension) (sky_trace.wgsl:799-800), so seeds that differ only in
+    their low bits hand the same random streams to nearby pixels or samples
+    (seed 2 gives pixel p the stream seed 1 gives pixel p ^ 3).  k << 48 puts
+    seed_hi = k * 65536: every render gets sample keys of its own for up to
+    65536 samples per pixel."""
+    return int(k) << 48
+
+
 def midpoint_grid(n: int, lo: float = 380.0, hi: float = 780.0) -> np.ndarray:
     """Midpoint wavelengths: 16 -> 392.5 + 25 i nm, 64 -> 383.125 + 6.25 i nm."""
     step = (hi - lo) / n

@@ -4,8 +4,9 @@ The target is a render at the true parameters (its own seed); the fit starts
 from perturbed parameters and runs Adam on the relative squared error of
 block-averaged images,
 with gradients from the path-replay adjoint (SkyRender: forward + replay with
-common random numbers).  Each step renders with a fresh seed, so the
-gradient is an unbiased estimate of the expected loss's gradient; Adam works
+common random numbers).  Each step renders with random streams of its own
+(independent_seed), so the gradient is an unbiased estimate of the expected
+loss's gradient; Adam works
 in log-parameter space because the parameters span many orders of magnitude
 (m^-1 coefficients ~1e-6, heights ~1-25 km, g and albedo ~0.1-1).  The
 majorant grid is rebuilt from the current parameters every step (forward and
@@ -21,7 +22,7 @@ from typing import Callable, Optional
 import numpy as np
 
 from .adjoint import params_vector, sky_render, with_params
-from .api import Context, RenderConfig, SceneData
+from .api import Context, RenderConfig, SceneData, independent_seed
 
 
 def _blocks(x, b: int):
@@ -59,7 +60,7 @@ def fit(scene: SceneData, config: RenderConfig, target_xyz, theta0, steps: int =
     tail = []
     t0 = time.perf_counter()
     for it in range(steps):
-        cfg = dataclasses.replace(config, seed=seed0 + it, majorant_table=None)
+        cfg = dataclasses.replace(config, seed=independent_seed(seed0 + it), majorant_table=None)
         opt.zero_grad()
         theta = sign * torch.exp(u)
         xyz = sky_render(theta, scene, cfg, context)
