@@ -152,6 +152,12 @@ SceneData load_scene_data(const std::string &dir, float sun_elevation_deg, float
     return s;
 }
 
+void set_launch_tuning(const sky_tuning &tuning)
+{
+    sky_status st = sky_context_set_tuning(shared_context(), &tuning);
+    if (st != SKY_OK) throw to_error(st);
+}
+
 Film render(const SceneData &scene, const RenderConfig &config)
 {
     sky_render_config c;

@@ -4,7 +4,8 @@
 // the load / render / output / total timing lines.  The default band set is
 // the reference's 41 bands (380..780 nm every 10 nm).  --bands N (N midpoint
 // bands over 380-780 nm), --max-depth and --fast are extensions (path depth
-// limit, carried optical depth + phase sampling).  --data-dir (default data)
+// limit; carried optical depth + phase sampling in 3 geometric majorant
+// shells with the launch shape tuned on MI355X).  --data-dir (default data)
 // loads the reference's scene tables when present (the tabulated medium).
 #include <chrono>
 #include <cmath>
@@ -61,7 +62,7 @@ int main(int argc, char **argv)
     uint32_t bands = 0;  // 0: the reference's 41-band grid
     float png_exposure = 0.05f;
     std::string out = "out", data_dir = "data";
-    bool data_dir_set = false;
+    bool data_dir_set = false, fast = false;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
         auto next = [&]() -> const char * {
@@ -86,7 +87,7 @@ int main(int argc, char **argv)
         else if (a == "--rebuild-rgb-only") rebuild = true;
         else if (a == "--bands") bands = (uint32_t)std::strtoul(next(), nullptr, 10);
         else if (a == "--max-depth") cfg.max_depth = (uint32_t)std::strtoul(next(), nullptr, 10);
-        else if (a == "--fast") cfg.flags |= SKY_FLAG_CARRIED_TAU | SKY_FLAG_PHASE_IS;
+        else if (a == "--fast") fast = true;
         else {
             usage();
             return 2;
@@ -119,9 +120,25 @@ int main(int argc, char **argv)
             return 0;
         }
 
+        if (fast) {
+            // the estimator and launch shape the autotuner picked on MI355X for
+            // 64-band panoramas (analytic medium) and the 41-band sky-view bake
+            // on the reference's tables (sky-tracer_amd/pysky/tuning_cache.json)
+            cfg.flags |= SKY_FLAG_CARRIED_TAU | SKY_FLAG_PHASE_IS;
+            cfg.majorant_layers = 3;
+            cfg.shell_growth = tabulated ? 6.0 : 4.0;
+            sky_tuning t{};
+            t.waves_per_cu = tabulated ? 32 : 24;
+            t.pixels_per_item = tabulated ? 8 : 4;
+            t.event_threshold = tabulated ? 24 : 8;
+            t.track_steps = 2;
+            t.occupancy = 6;
+            sky::set_launch_tuning(t);
+        }
         std::printf("config: gpu-integrator output=%s direct-light-samples=%u\n", sky::label(cfg.output_projection),
                     cfg.direct_light_samples);
         std::printf("medium: %s\n", tabulated ? ("tables from " + data_dir).c_str() : "analytic");
+        if (fast) std::printf("estimator: carried optical depth + phase sampling, 3 majorant shells\n");
         std::fflush(stdout);
         auto render0 = std::chrono::steady_clock::now();
         sky::Film film = sky::render(scene, cfg);

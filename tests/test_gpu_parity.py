@@ -203,3 +203,29 @@ def test_cli_renders(tmp_path):
     again = read_exr(tmp_path / "sky_rgb.exr")
     for c in "RGB":
         assert np.array_equal(rgb[c], again[c])
+
+
+def test_cli_fast_renders_the_tuned_estimator(gpu_context, tmp_path):
+    """sky_render --fast: carried optical depth and lobe sampling in 3 majorant
+    shells growing 4x with the tuned launch shape (the specialised kernels);
+    its band films equal the same render through pysky up to the order of the
+    float sums."""
+    import json
+    import os
+    import subprocess
+
+    from exr_png import read_exr
+
+    exe = os.path.join(os.path.dirname(pysky.LIB_PATH), "..", "bin", "sky_render")
+    args = ["--width", "64", "--height", "32", "--spp", "16", "--sun-elevation-deg", "5", "--bands", "16",
+            "--max-depth", "8", "--fast", "--out", str(tmp_path)]
+    r = subprocess.run([exe, *args], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "estimator: carried optical depth" in r.stdout
+    m = json.load(open(tmp_path / "asset.json"))
+    bands = np.stack([read_exr(tmp_path / f)["R"] for f in m["files"]["band_exrs"]], axis=-1)
+    scene = pysky.load_scene(5.0, wavelengths_nm=m["band_centers_nm"])
+    cfg = pysky.RenderConfig(width=64, height=32, spp=16, seed=0x5EC720260430, max_depth=8, rr_start_depth=4,
+                             majorant_layers=3, shell_growth=4.0, carried_tau=True, phase_importance_sampling=True)
+    film = pysky.render(scene, cfg, context=gpu_context).values().reshape(32, 64, 16)
+    np.testing.assert_allclose(bands, film, rtol=2e-5, atol=1e-7 * film.max())
