@@ -152,11 +152,11 @@ SceneData load_scene_data(const std::string &dir, float sun_elevation_deg, float
     return s;
 }
 
-void set_launch_tuning(const sky_tuning &tuning)
-{
-    sky_status st = sky_context_set_tuning(shared_context(), &tuning);
-    if (st != SKY_OK) throw to_error(st);
+namespace {
+sky_tuning g_tuning{};  // zero fields: the library's defaults
 }
+
+void set_launch_tuning(const sky_tuning &tuning) { g_tuning = tuning; }
 
 Film render(const SceneData &scene, const RenderConfig &config)
 {
@@ -188,7 +188,10 @@ Film render(const SceneData &scene, const RenderConfig &config)
     size_t npx = (size_t)c.width * c.height;
     std::vector<float> film(npx * wl.count), xyz(npx * 3);
     sky_render_stats stats{};
-    st = sky_render(shared_context(), &scene.planet, &atm, &wl, &c, film.data(), nullptr, xyz.data(), &stats);
+    sky_context *ctx = shared_context();
+    st = sky_context_set_tuning(ctx, &g_tuning);
+    if (st != SKY_OK) throw to_error(st);
+    st = sky_render(ctx, &scene.planet, &atm, &wl, &c, film.data(), nullptr, xyz.data(), &stats);
     if (st != SKY_OK) throw to_error(st);
     return Film(c.width, c.height, wl.count, std::move(film), std::move(xyz));
 }

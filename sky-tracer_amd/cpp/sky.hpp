@@ -121,8 +121,9 @@ SceneData load_scene_data(float sun_elevation_deg, float sun_azimuth_deg, uint32
 // context per process), returns the normalised film (and its XYZ image).
 Film render(const SceneData &scene, const RenderConfig &config);
 
-// Launch shape of the wave-pool integrator for later render() calls (the
-// process-wide context's sky_context_set_tuning); throws RenderError.
+// Launch shape of the wave-pool integrator for later render() calls, applied
+// to the process-wide context after a render's configuration is validated
+// (sky_context_set_tuning; an invalid shape makes render() throw).
 void set_launch_tuning(const sky_tuning &tuning);
 
 }  // namespace sky

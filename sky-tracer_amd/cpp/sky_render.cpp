@@ -120,20 +120,22 @@ int main(int argc, char **argv)
             return 0;
         }
 
+        // The launch shape the autotuner picked on MI355X for 64-band panoramas
+        // (analytic medium) and the 41-band sky-view bake on the reference's
+        // tables (sky-tracer_amd/pysky/tuning_cache.json); it changes only the
+        // order of the float sums.  --fast also switches the estimator and the
+        // majorant shells to the tuned ones.
+        sky_tuning t{};
+        t.waves_per_cu = tabulated ? 32 : 24;
+        t.pixels_per_item = tabulated ? 8 : 4;
+        t.event_threshold = tabulated ? 24 : 8;
+        t.track_steps = 2;
+        t.occupancy = 6;
+        sky::set_launch_tuning(t);
         if (fast) {
-            // the estimator and launch shape the autotuner picked on MI355X for
-            // 64-band panoramas (analytic medium) and the 41-band sky-view bake
-            // on the reference's tables (sky-tracer_amd/pysky/tuning_cache.json)
             cfg.flags |= SKY_FLAG_CARRIED_TAU | SKY_FLAG_PHASE_IS;
             cfg.majorant_layers = 3;
             cfg.shell_growth = tabulated ? 6.0 : 4.0;
-            sky_tuning t{};
-            t.waves_per_cu = tabulated ? 32 : 24;
-            t.pixels_per_item = tabulated ? 8 : 4;
-            t.event_threshold = tabulated ? 24 : 8;
-            t.track_steps = 2;
-            t.occupancy = 6;
-            sky::set_launch_tuning(t);
         }
         std::printf("config: gpu-integrator output=%s direct-light-samples=%u\n", sky::label(cfg.output_projection),
                     cfg.direct_light_samples);
