@@ -520,10 +520,12 @@ def main():
             args.shell_growth = p.get("shell_growth", 1.0) if args.shell_growth <= 0 else args.shell_growth
             out = bench_batch(args, ctx)
         else:
-            # the adjoint and the fit use the large-render shell tuning
+            # the adjoint and the fit use the large-render tuning (shells and launch
+            # shape; a sweep of the C5 forward + replay finds the same launch shape)
             p = autotune.tuned(pysky.SceneData(wavelengths_nm=list(pysky.midpoint_grid(64))),
                                pysky.RenderConfig(width=1024, height=512, carried_tau=True,
                                                   phase_importance_sampling=True))
+            autotune.apply(ctx, pysky.RenderConfig(), p)
             args.layers = args.layers or p["majorant_layers"]
             args.shell_growth = p.get("shell_growth", 1.0) if args.shell_growth <= 0 else args.shell_growth
             out = bench_adjoint(args, ctx) if args.workload == "c5" else run_fit(args, ctx)
