@@ -152,10 +152,10 @@ typedef struct sky_render_stats {
 /* Launch tuning of the wave-pool integrator (autotuner output); zero fields
  * mean "use the default". */
 typedef struct sky_tuning {
-    uint32_t waves_per_cu;    /* persistent waves per compute unit (16)        */
+    uint32_t waves_per_cu;    /* persistent waves per compute unit (24)        */
     uint32_t pixels_per_item; /* pixels per wave work item (128 / count)       */
-    uint32_t event_threshold; /* lanes waiting before events are handled (32)  */
-    uint32_t track_steps;     /* tracking micro-steps between handler passes (8) */
+    uint32_t event_threshold; /* lanes waiting before events are handled (8)   */
+    uint32_t track_steps;     /* tracking micro-steps between handler passes (2) */
     uint32_t occupancy;       /* 0: compiler's register budget; 4, 5, 6, 8: at least that many waves per SIMD */
 } sky_tuning;
 

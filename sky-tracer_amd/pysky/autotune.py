@@ -22,7 +22,8 @@ from .api import Context, RenderConfig, SceneData, render
 
 CACHE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_cache.json")
 
-DEFAULT = dict(majorant_layers=16, event_threshold=32, track_steps=8, occupancy=0, waves_per_cu=16,
+# the library's defaults for the launch shape (zero tuning fields), 16 uniform shells
+DEFAULT = dict(majorant_layers=16, event_threshold=8, track_steps=2, occupancy=0, waves_per_cu=24,
                pixels_per_item=0, shell_growth=1.0)
 SPACE = {
     "majorant_layers": [2, 3, 4, 5, 6, 7, 8, 12, 16, 24, 32, 64],
