@@ -365,15 +365,16 @@ __device__ __forceinline__ void track_step(const DevScene &s, const MajSrc &m, L
     }
 }
 
-// The tracking step of the FAST kernels (carried optical depth, lobe
-// sampling, delta-tracked shadow rays, tables in LDS): the same walk, draws
-// and arithmetic as track_step, written for SIMT.  The step's free flight is
-// drawn before the walk (the first draw of a step is made in its first
-// non-empty segment either way, so the random sequence is unchanged); each
-// trip of the walk either finds the collision inside the current segment or
-// crosses into the next one, with selects in place of branches and one exit
-// per outcome; the collision test is in select form too.
-template <bool ADJ>
+// The tracking step of the FAST kernels (delta-tracked shadow rays, tables in
+// LDS; CARRIED: carried optical depth, else a fresh draw per shell as the
+// reference): the same walk, draws and arithmetic as track_step, written for
+// SIMT.  The step's free flight is drawn before the walk (the first draw of a
+// step is made in its first non-empty segment either way, so the random
+// sequence is unchanged); each trip of the walk either finds the collision
+// inside the current segment or crosses into the next one, with selects in
+// place of branches and one exit per outcome; the collision test is in select
+// form too.
+template <bool ADJ, bool CARRIED>
 __device__ __forceinline__ void track_step_fast(const DevScene &s, const MajSrc &m, LaneCtx &c, AdjLane &a,
                                                 const AdjArgs &A, bool &wd)
 {
@@ -382,12 +383,18 @@ __device__ __forceinline__ void track_step_fast(const DevScene &s, const MajSrc 
     bool ended = false;
     for (;;) {
         const bool nonempty = c.t < c.seg_end;
-        const float seg_tau = c.maj * (c.seg_end - c.t);
-        const bool cross = !nonempty || c.tau >= seg_tau;
         tn = c.t + f_div(c.tau, c.maj);
-        if (!cross && tn < c.seg_end) break;
-        // a draw that lands at or past the segment end by rounding is spent
-        c.tau = cross ? (nonempty ? c.tau - seg_tau : c.tau) : -1.0f;
+        if constexpr (CARRIED) {
+            const float seg_tau = c.maj * (c.seg_end - c.t);
+            const bool cross = !nonempty || c.tau >= seg_tau;
+            if (!cross && tn < c.seg_end) break;
+            // a draw that lands at or past the segment end by rounding is spent
+            c.tau = cross ? (nonempty ? c.tau - seg_tau : c.tau) : -1.0f;
+        } else {
+            // every non-empty segment spends its own draw (sky_trace.wgsl:425-434)
+            if (nonempty && tn < c.seg_end) break;
+            c.tau = nonempty ? -1.0f : c.tau;
+        }
         c.t = c.seg_end;
         if ((c.t >= c.tmax) | (c.guard == 0u)) {
             ended = true;
@@ -684,13 +691,14 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
     if (nx == NX_FINISH) finish_path<ADJ>(c, a, acc_sum, acc_sq, acc_grad);
 }
 
-// handle_event for the FAST kernels (lobe sampling, delta-tracked shadow
-// rays, no adjoint): the same steps with every branch that has no side effect
+// handle_event for the FAST kernels (delta-tracked shadow rays; IS: Rayleigh/
+// HG lobe sampling, else isotropic directions weighted by the phase function
+// as the reference): the same steps with every branch that has no side effect
 // turned into selects, so a pass costs one evaluation of each piece instead of
 // the exec-mask bookkeeping of nested branches.  Same draws and arithmetic,
 // except that the four weight divisions multiply by the hardware reciprocal
 // (within 1 ulp of the quotient).
-template <bool ADJ>
+template <bool ADJ, bool IS>
 __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSrc &m, LaneCtx &c, AdjLane &a,
                                                   const AdjArgs &A, float *acc_sum, float *acc_sq, float *acc_grad,
                                                   bool &wd)
@@ -739,9 +747,11 @@ __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSr
     const bool scat_v = c.vtype == 0;
 
     // 2. this pass's random numbers in the reference's order: sun cone (2);
-    //    lobe choice, mu, azimuth (3) or cosine lobe (2), then the roulette draw
+    //    lobe choice, mu, azimuth (3), isotropic direction (2) or cosine lobe
+    //    (2), then the roulette draw
     const bool rr = c.depth >= s.rr_start;
-    const uint32_t ndraw = is_nee ? 2u : (is_cont ? (scat_v ? 3u : 2u) + (rr ? 1u : 0u) : 0u);
+    const bool lobe = IS && scat_v;
+    const uint32_t ndraw = is_nee ? 2u : (is_cont ? (lobe ? 3u : 2u) + (rr ? 1u : 0u) : 0u);
     const float u0 = rng_peek(c.rng, 0);
     const float u1 = rng_peek(c.rng, 1);
     float u2 = 0.0f, u3 = 0.0f;
@@ -749,19 +759,30 @@ __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSr
     if (ndraw > 3) u3 = rng_peek(c.rng, 3);
     c.rng.w += ndraw * kLcgMul;
 
-    // 3. one direction: sun cone (NEE), Rayleigh/HG lobe at a scattering
-    //    vertex, cosine lobe at the ground
+    // 3. one direction: sun cone (NEE), Rayleigh/HG lobe (IS) or isotropic
+    //    (sample_isotropic, wgsl:534-539, as a frame around +y with the
+    //    azimuth negated) at a scattering vertex, cosine lobe at the ground
     const f3 nrm = normalize_or_y(f3{(float)c.px, (float)c.py, (float)c.pz});
     float mu_sun, st_sun;
     sun_cone_mu(s, u0, mu_sun, st_sun);
-    const float mu_lobe = u0 < c.rw ? sample_rayleigh_mu(u1) : sample_hg_mu(s.g, u1);
-    const float st_lobe = f_sqrt(fmaxf(1.0f - mu_lobe * mu_lobe, 0.0f));
+    float mu_v0, phi_v;
+    f3 axis_v;
+    if constexpr (IS) {
+        mu_v0 = u0 < c.rw ? sample_rayleigh_mu(u1) : sample_hg_mu(s.g, u1);
+        axis_v = c.vin;
+        phi_v = u2;
+    } else {
+        mu_v0 = 1.0f - 2.0f * u0;
+        axis_v = f3{0.0f, 1.0f, 0.0f};
+        phi_v = -u1;
+    }
+    const float st_v = f_sqrt(fmaxf(1.0f - mu_v0 * mu_v0, 0.0f));
     const float mu_cos = f_sqrt(fmaxf(1.0f - u0, 0.0f)), st_cos = f_sqrt(u0);
-    const float mu = is_nee ? mu_sun : (scat_v ? mu_lobe : mu_cos);
-    const float st = is_nee ? st_sun : (scat_v ? st_lobe : st_cos);
-    const float phi = (!is_nee && scat_v) ? u2 : u1;
+    const float mu = is_nee ? mu_sun : (scat_v ? mu_v0 : mu_cos);
+    const float st = is_nee ? st_sun : (scat_v ? st_v : st_cos);
+    const float phi = (!is_nee && scat_v) ? phi_v : u1;
     const f3 sun{s.sun[0], s.sun[1], s.sun[2]};
-    const f3 axis = is_nee ? sun : (scat_v ? c.vin : (c.vtype == 1 ? nrm : f3{0.0f, 1.0f, 0.0f}));
+    const f3 axis = is_nee ? sun : (scat_v ? axis_v : (c.vtype == 1 ? nrm : f3{0.0f, 1.0f, 0.0f}));
     const f3 dir = (is_nee || is_cont) ? frame_dir(axis, mu, st, phi) : c.vin;
 
     // 4. weights.  NEE: the sun sample's factor; a ground facing away from
@@ -776,9 +797,11 @@ __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSr
     // for the cosine lobe) and the roulette (wgsl:691-701, 751-770)
     const float Lc = c.L + c.tp * f_div(c.nee_sum, (float)n_light);
     const bool ok = scat_v || cs > 0.0f;
-    const float tp1 = c.tp * (scat_v ? 1.0f : s.albedo);
+    // bounce weight: 1 for a lobe sample, phase / isotropic pdf for an
+    // isotropic one, the albedo for the cosine lobe
+    const float tp1 = c.tp * (scat_v ? (IS ? 1.0f : ph * (1.0f / kIsotropicPdf)) : s.albedo);
     const float q = fminf(fmaxf(tp1, 0.05f), 0.95f);
-    const bool killed = rr && (scat_v ? u3 : u2) > q;
+    const bool killed = rr && (lobe ? u3 : u2) > q;
     const float tp2 = rr ? f_div(tp1, q) : tp1;
     const bool live = ok && !killed;
     const bool bad = Lc < 0.0f || Lc != Lc || tp2 < 0.0f || tp2 != tp2;
@@ -946,9 +969,13 @@ __device__ __forceinline__ uint32_t find_job(const PoolArgs &a, uint32_t item)
     return __builtin_amdgcn_readfirstlane(lo);
 }
 
-template <bool ADJ, bool FAST>
+// FK: 0 the general kernels; 1 FAST with carried optical depth and lobe
+// sampling; 2 FAST with the reference's estimator (a fresh draw per shell,
+// isotropic directions).
+template <bool ADJ, int FK>
 __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &A)
 {
+    constexpr bool FAST = FK != 0;
     extern __shared__ float dyn_lds[];
     uint32_t loaded = 0xFFFFFFFFu;  // job whose majorants are in the wave's LDS copy
     const uint32_t wid = threadIdx.x >> 6;
@@ -1034,7 +1061,8 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
             if ((evm || ndm) && ((uint32_t)__popcll(evm | ndm) >= a.event_threshold || !trm)) {
                 // handler pass: events, refill of finished lanes, ray set-ups
                 if constexpr (FAST) {
-                    if (c.st == ST_EVENT) handle_event_fast<ADJ>(s, msrc, c, al, A, acc_sum, acc_sq, acc_grad, wd);
+                    if (c.st == ST_EVENT)
+                        handle_event_fast<ADJ, FK == 1>(s, msrc, c, al, A, acc_sum, acc_sq, acc_grad, wd);
                 } else {
                     if (c.st == ST_EVENT) handle_event<ADJ>(s, msrc, c, al, A, acc_sum, acc_sq, acc_grad, wd);
                 }
@@ -1055,7 +1083,7 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
                 if (!ballot(c.st == ST_TRACK)) break;
                 if (c.st == ST_TRACK) {
                     if constexpr (FAST)
-                        track_step_fast<ADJ>(s, msrc, c, al, A, wd);
+                        track_step_fast<ADJ, FK == 1>(s, msrc, c, al, A, wd);
                     else
                         track_step<ADJ>(s, msrc, c, al, A, wd);
                 }
@@ -1082,31 +1110,33 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
 // allocation and caps for at least 4, 5, 6 or 8 waves per SIMD (512 VGPRs per
 // lane / waves, 8-register granules: 128, 96, 80, 64).  The autotuner picks
 // one per workload.  The _fast kernels serve launches whose jobs all use
-// carried optical depth, lobe sampling and delta-tracked shadow rays with
-// their tables in LDS.
-__global__ __launch_bounds__(256) void k_wavepool(PoolArgs a) { wavepool_body<false, false>(a, AdjArgs{}); }
-__global__ __launch_bounds__(256) void k_wavepool_fast(PoolArgs a) { wavepool_body<false, true>(a, AdjArgs{}); }
+// carried optical depth, lobe sampling and delta-tracked shadow rays, the
+// _fastref kernels launches whose jobs all use the reference's estimator
+// (a fresh draw per shell, isotropic directions, delta-tracked shadow rays),
+// both with at most kFastMaxShells shells and their tables in LDS.
+__global__ __launch_bounds__(256) void k_wavepool(PoolArgs a) { wavepool_body<false, 0>(a, AdjArgs{}); }
+__global__ __launch_bounds__(256) void k_wavepool_fast(PoolArgs a) { wavepool_body<false, 1>(a, AdjArgs{}); }
+__global__ __launch_bounds__(256) void k_wavepool_fastref(PoolArgs a) { wavepool_body<false, 2>(a, AdjArgs{}); }
 
-#define SKY_WAVEPOOL_OCC(N)                                                                               \
-    __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N))) void k_wavepool_o##N(      \
-        PoolArgs a)                                                                                       \
-    {                                                                                                     \
-        wavepool_body<false, false>(a, AdjArgs{});                                                        \
-    }                                                                                                     \
-    __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N))) void k_wavepool_fast_o##N( \
-        PoolArgs a)                                                                                       \
-    {                                                                                                     \
-        wavepool_body<false, true>(a, AdjArgs{});                                                         \
+#define SKY_WAVEPOOL_OCC(N, NAME, FK)                                                                        \
+    __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N))) void NAME##_o##N(PoolArgs a) \
+    {                                                                                                        \
+        wavepool_body<false, FK>(a, AdjArgs{});                                                              \
     }
-SKY_WAVEPOOL_OCC(4)
-SKY_WAVEPOOL_OCC(5)
-SKY_WAVEPOOL_OCC(6)
-SKY_WAVEPOOL_OCC(8)
+#define SKY_WAVEPOOL_OCC3(N)             \
+    SKY_WAVEPOOL_OCC(N, k_wavepool, 0)   \
+    SKY_WAVEPOOL_OCC(N, k_wavepool_fast, 1) \
+    SKY_WAVEPOOL_OCC(N, k_wavepool_fastref, 2)
+SKY_WAVEPOOL_OCC3(4)
+SKY_WAVEPOOL_OCC3(5)
+SKY_WAVEPOOL_OCC3(6)
+SKY_WAVEPOOL_OCC3(8)
+#undef SKY_WAVEPOOL_OCC3
 #undef SKY_WAVEPOOL_OCC
 
 // Path-replay adjoint: the forward state machine plus the score bookkeeping.
-__global__ __launch_bounds__(256) void k_wavepool_adjoint(PoolArgs a, AdjArgs A) { wavepool_body<true, false>(a, A); }
-__global__ __launch_bounds__(256) void k_wavepool_adjoint_fast(PoolArgs a, AdjArgs A) { wavepool_body<true, true>(a, A); }
+__global__ __launch_bounds__(256) void k_wavepool_adjoint(PoolArgs a, AdjArgs A) { wavepool_body<true, 0>(a, A); }
+__global__ __launch_bounds__(256) void k_wavepool_adjoint_fast(PoolArgs a, AdjArgs A) { wavepool_body<true, 1>(a, A); }
 
 // Per job (blockIdx.y): reduces the chunk partials in chunk order,
 // normalises by 1/spp and clamps (integrator.rs:307-310), and forms
@@ -1191,8 +1221,9 @@ WavePlan plan_launch(const sky_tuning &t, int num_cus, uint32_t total_items, uin
 {
     WavePlan p;
     p.maj_lds = max_maj <= kMajLdsMax ? (max_maj + 63u) & ~63u : 0u;
-    const uint32_t fast_flags = kFlagCarriedTau | kFlagPhaseIS;
-    p.fast = p.maj_lds != 0 && max_layers <= kFastMaxShells && (flags_all & (fast_flags | kFlagRatio)) == fast_flags;
+    const uint32_t est = flags_all & (kFlagCarriedTau | kFlagPhaseIS | kFlagRatio);
+    const bool fits = p.maj_lds != 0 && max_layers <= kFastMaxShells && flags_all != kFlagMixed;
+    p.fast = !fits ? 0 : est == (kFlagCarriedTau | kFlagPhaseIS) ? 1 : est == 0 ? 2 : 0;
     p.shell_lds = p.maj_lds ? ((max_layers + 1u + 7u) & ~7u) : 0u;
     p.lam_lds = p.fast ? 5u * (uint32_t)kMaxLambda : 0u;
     p.lds_bytes = kWavesPerBlock * (kLdsFixed + p.maj_lds + 2 * p.shell_lds + p.lam_lds) * (uint32_t)sizeof(float);
@@ -1232,14 +1263,13 @@ hipError_t launch_pool(const DevJob *d_jobs, uint32_t n_jobs, uint32_t max_npx, 
     a.shell_lds = p.shell_lds;
     a.lam_lds = p.lam_lds;
     typedef void (*PoolKernel)(PoolArgs);
-    PoolKernel k;
-    switch (p.occupancy) {
-    case 4: k = p.fast ? k_wavepool_fast_o4 : k_wavepool_o4; break;
-    case 5: k = p.fast ? k_wavepool_fast_o5 : k_wavepool_o5; break;
-    case 6: k = p.fast ? k_wavepool_fast_o6 : k_wavepool_o6; break;
-    case 8: k = p.fast ? k_wavepool_fast_o8 : k_wavepool_o8; break;
-    default: k = p.fast ? k_wavepool_fast : k_wavepool; break;
-    }
+    static const PoolKernel kernels[3][5] = {
+        {k_wavepool, k_wavepool_o4, k_wavepool_o5, k_wavepool_o6, k_wavepool_o8},
+        {k_wavepool_fast, k_wavepool_fast_o4, k_wavepool_fast_o5, k_wavepool_fast_o6, k_wavepool_fast_o8},
+        {k_wavepool_fastref, k_wavepool_fastref_o4, k_wavepool_fastref_o5, k_wavepool_fastref_o6,
+         k_wavepool_fastref_o8}};
+    const int oi = p.occupancy == 4 ? 1 : p.occupancy == 5 ? 2 : p.occupancy == 6 ? 3 : p.occupancy == 8 ? 4 : 0;
+    const PoolKernel k = kernels[p.fast][oi];
     hipLaunchKernelGGL(k, dim3(p.grid), dim3(p.block), p.lds_bytes, stream, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -1262,7 +1292,7 @@ hipError_t launch_adjoint(const DevJob *d_jobs, const WavePlan &p, uint32_t *met
     a.maj_lds = p.maj_lds;
     a.shell_lds = p.shell_lds;
     a.lam_lds = p.lam_lds;
-    hipLaunchKernelGGL(p.fast ? k_wavepool_adjoint_fast : k_wavepool_adjoint, dim3(p.grid), dim3(p.block),
+    hipLaunchKernelGGL(p.fast == 1 ? k_wavepool_adjoint_fast : k_wavepool_adjoint, dim3(p.grid), dim3(p.block),
                        p.lds_bytes, stream, a, A);
     return hipGetLastError();
 }

@@ -1,9 +1,11 @@
-"""The specialised integrator kernels (k_wavepool_fast*: carried optical depth,
-lobe sampling, delta-tracked shadow rays, at most 8 majorant shells held in
-LDS) against the general kernels on the same paths.
+"""The specialised integrator kernels (k_wavepool_fast*: carried optical depth
+and lobe sampling; k_wavepool_fastref*: the reference's estimator, a fresh
+draw per shell and isotropic directions; both with delta-tracked shadow rays
+and at most 8 majorant shells held in LDS) against the general kernels on the
+same paths.
 
-A single render with that estimator launches a specialised kernel; a batch
-whose jobs use different estimator flags launches the general one.  Paths are
+A single render with one of those estimators launches a specialised kernel; a
+batch whose jobs use different estimator flags launches the general one.  Paths are
 keyed the same way in both, so the films agree up to the order of the float
 sums and the specialised handler's reciprocal weight divisions (within 1 ulp
 of the quotients): entry by entry to 1e-4, except where a roulette or accept
@@ -24,27 +26,29 @@ DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "refer
 
 
 def _fast_and_general(ctx, scene, cfg):
-    other = dataclasses.replace(cfg, carried_tau=False, phase_importance_sampling=False, width=8, height=4,
-                                spp=1)
+    other = dataclasses.replace(cfg, carried_tau=not cfg.carried_tau, phase_importance_sampling=False, width=8,
+                                height=4, spp=1)
     general = pysky.render_batch([(scene, cfg), (scene, other)], want_film=True, use_graph=False,
                                  context=ctx).films[0].values()
     fast = pysky.render(scene, cfg, context=ctx).values()
     return fast, general
 
 
-def _analytic(sun, albedo, layers, growth):
+def _analytic(sun, albedo, layers, growth, reference=False):
     atm = pysky.AtmosphereParams(ground_albedo=albedo)
     scene = pysky.SceneData(atmosphere=atm, sun_elevation_deg=sun, wavelengths_nm=list(pysky.midpoint_grid(16)))
     cfg = pysky.RenderConfig(width=64, height=32, spp=32, seed=3, max_depth=16, rr_start_depth=4,
-                             majorant_layers=layers, shell_growth=growth, carried_tau=True,
-                             phase_importance_sampling=True)
+                             majorant_layers=layers, shell_growth=growth, carried_tau=not reference,
+                             phase_importance_sampling=not reference)
     return scene, cfg
 
 
+@pytest.mark.parametrize("reference", [False, True])
 @pytest.mark.parametrize("sun,albedo,layers,growth", [(5.0, 0.3, 3, 4.0), (-4.0, 0.8, 8, 1.0), (30.0, 0.3, 1, 1.0)])
-def test_fast_kernel_traces_the_general_kernels_paths(gpu_context, sun, albedo, layers, growth):
-    scene, cfg = _analytic(sun, albedo, layers, growth)
-    print(f"[gpu] fast vs general: sun {sun} albedo {albedo} shells {layers} growth {growth}", flush=True)
+def test_fast_kernel_traces_the_general_kernels_paths(gpu_context, sun, albedo, layers, growth, reference):
+    scene, cfg = _analytic(sun, albedo, layers, growth, reference)
+    print(f"[gpu] fast vs general: sun {sun} albedo {albedo} shells {layers} growth {growth} "
+          f"{'reference estimator' if reference else 'carried + lobe'}", flush=True)
     fast, general = _fast_and_general(gpu_context, scene, cfg)
     frac, bad = paired_stats(fast, general, rtol=1e-4)
     print(f"paired fraction {frac:.6f} ({len(bad)} of {fast.size} differ)")
