@@ -148,6 +148,15 @@ def test_watchdog_segment_guard(gpu_context):
         _gpu(scene, cfg, gpu_context)
 
 
+@pytest.mark.parametrize("mode", ["carried_is", "reference"])
+def test_watchdog_segment_guard_in_specialised_kernels(gpu_context, mode):
+    """The specialised kernels (3 shells) count the crossing budget down; a
+    zero budget trips the watchdog as in the general kernel."""
+    scene, cfg, _ = case(width=16, height=8, spp=2, mode=mode, layers=3, shell_growth=4.0, max_segments=0)
+    with pytest.raises(pysky.Watchdog):
+        _gpu(scene, cfg, gpu_context)
+
+
 def test_invalid_config(gpu_context):
     scene, cfg, _ = case(width=16, height=8, spp=0)
     with pytest.raises(pysky.InvalidConfig):
