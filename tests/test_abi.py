@@ -154,3 +154,21 @@ def test_cli_reports_invalid_config(tmp_path):
                        capture_output=True, text=True, timeout=60)
     assert r.returncode == 1
     assert "greater than zero" in r.stderr
+
+
+def test_independent_seeds_give_disjoint_sample_keys():
+    """A path's sample key is sample ^ seed_hi (sky_trace.wgsl:799-800): the
+    seeds of pysky.independent_seed give every render sample keys of its own
+    for up to 65536 samples per pixel, and leave the pixel keys alone."""
+    spp = 65536
+    keys = []
+    for k in range(1, 5):
+        seed = pysky.independent_seed(k)
+        assert seed & 0xFFFFFFFF == 0
+        hi = seed >> 32
+        keys.append({s ^ hi for s in range(0, spp, 997)} | {(spp - 1) ^ hi})
+    for i in range(len(keys)):
+        for j in range(i + 1, len(keys)):
+            assert not keys[i] & keys[j]
+    # seeds 1 and 2, by contrast, hand the same set of pixel keys to a 64-pixel image
+    assert {p ^ 1 for p in range(64)} == {p ^ 2 for p in range(64)}
