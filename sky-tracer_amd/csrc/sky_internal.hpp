@@ -103,7 +103,7 @@ struct WavePlan {
     uint32_t maj_lds = 0;   // floats per wave for the majorant grid in LDS (0: global)
     uint32_t shell_lds = 0; // doubles per wave for the shell table in LDS (0 with maj_lds 0)
     uint32_t lds_bytes = 0; // dynamic LDS per block
-    int fast = 0;           // 1: the _fast kernels, 2: the _fastref kernels (see plan_launch)
+    int fast = 0;           // 1: the _fast, 2: the _fastref, 3: the _fastrt kernels (see plan_launch)
     uint32_t lam_lds = 0;   // floats per wave for the per-band coefficients in LDS (fast kernels)
 };
 
@@ -114,9 +114,10 @@ uint32_t launch_min_items(const sky_tuning &t, int num_cus);
 // max_layers: the largest majorant_layers; flags_all: the estimator flags if
 // every job has the same ones, else kFlagMixed.  The _fast kernels serve
 // carried optical depth + lobe sampling + delta-tracked shadow rays, the
-// _fastref kernels the reference's estimator (fresh draw per shell,
-// isotropic directions, delta-tracked shadow rays), both with the tables in
-// LDS and at most kFastMaxShells majorant shells.
+// _fastrt kernels the same with ratio-tracked shadow rays, the _fastref
+// kernels the reference's estimator (fresh draw per shell, isotropic
+// directions, delta-tracked shadow rays), all with the tables in LDS and at
+// most kFastMaxShells majorant shells.
 constexpr uint32_t kFlagMixed = 0x80000000u;
 constexpr uint32_t kFastMaxShells = 8;
 WavePlan plan_launch(const sky_tuning &t, int num_cus, uint32_t total_items, uint32_t max_maj,

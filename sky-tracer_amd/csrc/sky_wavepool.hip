@@ -365,16 +365,16 @@ __device__ __forceinline__ void track_step(const DevScene &s, const MajSrc &m, L
     }
 }
 
-// The tracking step of the FAST kernels (delta-tracked shadow rays, tables in
-// LDS; CARRIED: carried optical depth, else a fresh draw per shell as the
-// reference): the same walk, draws and arithmetic as track_step, written for
-// SIMT.  The step's free flight is drawn before the walk (the first draw of a
+// The tracking step of the FAST kernels (tables in LDS; CARRIED: carried
+// optical depth, else a fresh draw per shell as the reference; RATIO: ratio-
+// tracked shadow rays, else delta-tracked): the same walk, draws and
+// arithmetic as track_step, written for SIMT.  The step's free flight is drawn before the walk (the first draw of a
 // step is made in its first non-empty segment either way, so the random
 // sequence is unchanged); each trip of the walk either finds the collision
 // inside the current segment or crosses into the next one, with selects in
 // place of branches and one exit per outcome; the collision test is in select
 // form too.
-template <bool ADJ, bool CARRIED>
+template <bool ADJ, bool CARRIED, bool RATIO>
 __device__ __forceinline__ void track_step_fast(const DevScene &s, const MajSrc &m, LaneCtx &c, AdjLane &a,
                                                 const AdjArgs &A, bool &wd)
 {
@@ -409,16 +409,19 @@ __device__ __forceinline__ void track_step_fast(const DevScene &s, const MajSrc 
     // The end of the ray, or the tentative collision and its accept test
     // (with the albedo weight of an accepted main-ray collision), in
     // select form: the collision's draw is taken only by lanes that have one.
+    // A ratio-tracked shadow ray instead multiplies its transmittance by
+    // 1 - sigma_t / majorant and goes on (sky_trace.wgsl:441-475).
     const bool col = !ended;
     const bool main = c.mode == kTrackCollision;
+    const bool rt = RATIO && c.mode == kTrackRatio;
     // (an ended lane evaluates the medium at its end point, unused)
     const float h = lane_altitude(s, c, col ? tn : c.t);
     const Coeffs co = lane_medium<true>(s, m, c, h);
     const float st = co.sr + co.sas + co.so + co.saa;
     const float ratio = fminf(fmaxf(f_div(st, c.maj), 0.0f), 1.0f);
     const float u = rng_peek(c.rng, 0);
-    c.rng.w += col ? kLcgMul : 0u;
-    const bool acc = col && u < ratio;
+    c.rng.w += col && !rt ? kLcgMul : 0u;
+    const bool acc = col && !rt && u < ratio;
     const float scat = co.sr + co.sas;
     const bool pos = st > 0.0f && scat > 0.0f;
     const float tpw = pos ? c.tp * fminf(fmaxf(f_div(scat, st), 0.0f), 1.0f) : 0.0f;
@@ -430,7 +433,7 @@ __device__ __forceinline__ void track_step_fast(const DevScene &s, const MajSrc 
     c.ev = ended ? (main ? EV_BOUNDARY : EV_SHADOW) : (acc ? (main ? EV_HIT : EV_SHADOW) : c.ev);
     c.tp = acc && main ? tpw : c.tp;
     c.rw = acc && main && pos ? f_div(co.sr, scat) : c.rw;
-    c.aux = dog || (acc && !main) ? 0.0f : c.aux;
+    c.aux = dog || (acc && !main) ? 0.0f : (rt && col ? c.aux * (1.0f - ratio) : c.aux);
     c.kind = dog ? 2 : c.kind;
     if constexpr (ADJ) {
         // accepted main-ray collision: d log(sigma_s / mu) into S; null
@@ -691,14 +694,14 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
     if (nx == NX_FINISH) finish_path<ADJ>(c, a, acc_sum, acc_sq, acc_grad);
 }
 
-// handle_event for the FAST kernels (delta-tracked shadow rays; IS: Rayleigh/
-// HG lobe sampling, else isotropic directions weighted by the phase function
-// as the reference): the same steps with every branch that has no side effect
+// handle_event for the FAST kernels (IS: Rayleigh/HG lobe sampling, else
+// isotropic directions weighted by the phase function as the reference;
+// RATIO: ratio-tracked shadow rays, else delta-tracked): the same steps with every branch that has no side effect
 // turned into selects, so a pass costs one evaluation of each piece instead of
 // the exec-mask bookkeeping of nested branches.  Same draws and arithmetic,
 // except that the four weight divisions multiply by the hardware reciprocal
 // (within 1 ulp of the quotient).
-template <bool ADJ, bool IS>
+template <bool ADJ, bool IS, bool RATIO>
 __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSrc &m, LaneCtx &c, AdjLane &a,
                                                   const AdjArgs &A, float *acc_sum, float *acc_sq, float *acc_grad,
                                                   bool &wd)
@@ -849,7 +852,7 @@ __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSr
     if (need_ray) {
         c.st = ST_RAY;
         c.d = dir;
-        c.mode = is_nee ? kTrackDelta : kTrackCollision;
+        c.mode = is_nee ? (RATIO ? kTrackRatio : kTrackDelta) : kTrackCollision;
         c.nee_factor = is_nee ? factor : c.nee_factor;
     }
     if (nx == NX_FINISH) finish_path<ADJ>(c, a, acc_sum, acc_sq, acc_grad);
@@ -971,7 +974,7 @@ __device__ __forceinline__ uint32_t find_job(const PoolArgs &a, uint32_t item)
 
 // FK: 0 the general kernels; 1 FAST with carried optical depth and lobe
 // sampling; 2 FAST with the reference's estimator (a fresh draw per shell,
-// isotropic directions).
+// isotropic directions); 3 as 1 with ratio-tracked shadow rays.
 template <bool ADJ, int FK>
 __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &A)
 {
@@ -1062,7 +1065,8 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
                 // handler pass: events, refill of finished lanes, ray set-ups
                 if constexpr (FAST) {
                     if (c.st == ST_EVENT)
-                        handle_event_fast<ADJ, FK == 1>(s, msrc, c, al, A, acc_sum, acc_sq, acc_grad, wd);
+                        handle_event_fast<ADJ, FK == 1 || FK == 3, FK == 3>(s, msrc, c, al, A, acc_sum, acc_sq,
+                                                                            acc_grad, wd);
                 } else {
                     if (c.st == ST_EVENT) handle_event<ADJ>(s, msrc, c, al, A, acc_sum, acc_sq, acc_grad, wd);
                 }
@@ -1083,7 +1087,7 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
                 if (!ballot(c.st == ST_TRACK)) break;
                 if (c.st == ST_TRACK) {
                     if constexpr (FAST)
-                        track_step_fast<ADJ, FK == 1>(s, msrc, c, al, A, wd);
+                        track_step_fast<ADJ, FK == 1 || FK == 3, FK == 3>(s, msrc, c, al, A, wd);
                     else
                         track_step<ADJ>(s, msrc, c, al, A, wd);
                 }
@@ -1113,20 +1117,24 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
 // carried optical depth, lobe sampling and delta-tracked shadow rays, the
 // _fastref kernels launches whose jobs all use the reference's estimator
 // (a fresh draw per shell, isotropic directions, delta-tracked shadow rays),
-// both with at most kFastMaxShells shells and their tables in LDS.
+// the _fastrt kernels carried optical depth + lobe sampling with ratio-
+// tracked shadow rays, all with at most kFastMaxShells shells and their
+// tables in LDS.
 __global__ __launch_bounds__(256) void k_wavepool(PoolArgs a) { wavepool_body<false, 0>(a, AdjArgs{}); }
 __global__ __launch_bounds__(256) void k_wavepool_fast(PoolArgs a) { wavepool_body<false, 1>(a, AdjArgs{}); }
 __global__ __launch_bounds__(256) void k_wavepool_fastref(PoolArgs a) { wavepool_body<false, 2>(a, AdjArgs{}); }
+__global__ __launch_bounds__(256) void k_wavepool_fastrt(PoolArgs a) { wavepool_body<false, 3>(a, AdjArgs{}); }
 
 #define SKY_WAVEPOOL_OCC(N, NAME, FK)                                                                        \
     __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N))) void NAME##_o##N(PoolArgs a) \
     {                                                                                                        \
         wavepool_body<false, FK>(a, AdjArgs{});                                                              \
     }
-#define SKY_WAVEPOOL_OCC3(N)             \
-    SKY_WAVEPOOL_OCC(N, k_wavepool, 0)   \
+#define SKY_WAVEPOOL_OCC3(N)                \
+    SKY_WAVEPOOL_OCC(N, k_wavepool, 0)      \
     SKY_WAVEPOOL_OCC(N, k_wavepool_fast, 1) \
-    SKY_WAVEPOOL_OCC(N, k_wavepool_fastref, 2)
+    SKY_WAVEPOOL_OCC(N, k_wavepool_fastref, 2) \
+    SKY_WAVEPOOL_OCC(N, k_wavepool_fastrt, 3)
 SKY_WAVEPOOL_OCC3(4)
 SKY_WAVEPOOL_OCC3(5)
 SKY_WAVEPOOL_OCC3(6)
@@ -1223,7 +1231,11 @@ WavePlan plan_launch(const sky_tuning &t, int num_cus, uint32_t total_items, uin
     p.maj_lds = max_maj <= kMajLdsMax ? (max_maj + 63u) & ~63u : 0u;
     const uint32_t est = flags_all & (kFlagCarriedTau | kFlagPhaseIS | kFlagRatio);
     const bool fits = p.maj_lds != 0 && max_layers <= kFastMaxShells && flags_all != kFlagMixed;
-    p.fast = !fits ? 0 : est == (kFlagCarriedTau | kFlagPhaseIS) ? 1 : est == 0 ? 2 : 0;
+    p.fast = !fits                                                  ? 0
+             : est == (kFlagCarriedTau | kFlagPhaseIS)                ? 1
+             : est == 0                                               ? 2
+             : est == (kFlagCarriedTau | kFlagPhaseIS | kFlagRatio) ? 3
+                                                                      : 0;
     p.shell_lds = p.maj_lds ? ((max_layers + 1u + 7u) & ~7u) : 0u;
     p.lam_lds = p.fast ? 5u * (uint32_t)kMaxLambda : 0u;
     p.lds_bytes = kWavesPerBlock * (kLdsFixed + p.maj_lds + 2 * p.shell_lds + p.lam_lds) * (uint32_t)sizeof(float);
@@ -1263,11 +1275,12 @@ hipError_t launch_pool(const DevJob *d_jobs, uint32_t n_jobs, uint32_t max_npx, 
     a.shell_lds = p.shell_lds;
     a.lam_lds = p.lam_lds;
     typedef void (*PoolKernel)(PoolArgs);
-    static const PoolKernel kernels[3][5] = {
+    static const PoolKernel kernels[4][5] = {
         {k_wavepool, k_wavepool_o4, k_wavepool_o5, k_wavepool_o6, k_wavepool_o8},
         {k_wavepool_fast, k_wavepool_fast_o4, k_wavepool_fast_o5, k_wavepool_fast_o6, k_wavepool_fast_o8},
         {k_wavepool_fastref, k_wavepool_fastref_o4, k_wavepool_fastref_o5, k_wavepool_fastref_o6,
-         k_wavepool_fastref_o8}};
+         k_wavepool_fastref_o8},
+        {k_wavepool_fastrt, k_wavepool_fastrt_o4, k_wavepool_fastrt_o5, k_wavepool_fastrt_o6, k_wavepool_fastrt_o8}};
     const int oi = p.occupancy == 4 ? 1 : p.occupancy == 5 ? 2 : p.occupancy == 6 ? 3 : p.occupancy == 8 ? 4 : 0;
     const PoolKernel k = kernels[p.fast][oi];
     hipLaunchKernelGGL(k, dim3(p.grid), dim3(p.block), p.lds_bytes, stream, a);

@@ -41,7 +41,8 @@ def test_paths_pair_with_oracle(gpu_context, mode, simple):
     assert np.all(np.isfinite(film)) and np.all(film >= 0)
 
 
-@pytest.mark.parametrize("mode,layers,growth", [("carried_is", 3, 3.0), ("reference", 7, 1.6), ("carried", 5, 2.5)])
+@pytest.mark.parametrize("mode,layers,growth", [("carried_is", 3, 3.0), ("reference", 7, 1.6), ("carried", 5, 2.5),
+                                               ("carried_is_ratio", 3, 4.0)])
 @pytest.mark.parametrize("simple", [False, True])
 def test_growing_shells_pair_with_oracle(gpu_context, mode, layers, growth, simple):
     """Non-uniform majorant shells (thickness growing by `growth`): the shell
@@ -51,7 +52,8 @@ def test_growing_shells_pair_with_oracle(gpu_context, mode, layers, growth, simp
     film = _gpu(scene, cfg, gpu_context).values()
     ref, _, _, _, rc = O.render(op, "f32")
     assert rc == 0
-    frac, bad = paired_stats(film, ref.ravel(), rtol=1e-3)
+    # ratio-tracked shadow rays carry a continuous transmittance product (see above)
+    frac, bad = paired_stats(film, ref.ravel(), rtol=1e-2 if mode == "carried_is_ratio" else 1e-3)
     print(f"paired fraction {frac:.5f} ({len(bad)} of {film.size} differ)")
     assert frac >= 0.995
 
