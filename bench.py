@@ -46,14 +46,15 @@ SCAN_TRAFFIC_BYTES = {6: (2.0 * 5189454.0 + 70256.1) * 1024}
 
 def pool_kernel(mode, params, n_lambda):
     """Name of the integrator kernel a render launches: the library picks the
-    _fast variants for carried optical depth + lobe sampling and the _fastref
-    variants for the reference's estimator, both with delta-tracked shadow
-    rays and at most 8 majorant shells whose grid fits its LDS copy
+    _fast (_fastrt) variants for carried optical depth + lobe sampling with
+    delta- (ratio-) tracked shadow rays and the _fastref variants for the
+    reference's estimator, all with at most 8 majorant shells whose grid fits
+    its LDS copy
     (n_lambda x shells <= 2048), and the register-budget variant from the
     tuning's occupancy."""
     layers = params["majorant_layers"]
     fits = layers <= 8 and n_lambda * layers <= 2048
-    kind = {"carried_is": "_fast", "reference": "_fastref"}.get(mode, "") if fits else ""
+    kind = {"carried_is": "_fast", "carried_is_ratio": "_fastrt", "reference": "_fastref"}.get(mode, "") if fits else ""
     occ = params.get("occupancy", 0)
     return "k_wavepool" + kind + (f"_o{occ}" if occ else "")
 
