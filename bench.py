@@ -54,7 +54,8 @@ def pool_kernel(mode, params, n_lambda):
     tuning's occupancy."""
     layers = params["majorant_layers"]
     fits = layers <= 8 and n_lambda * layers <= 2048
-    kind = {"carried_is": "_fast", "carried_is_ratio": "_fastrt", "reference": "_fastref"}.get(mode, "") if fits else ""
+    kinds = {"carried_is": "_fast", "carried_is_ratio": "_fastrt", "reference": "_fastref"}
+    kind = kinds.get(mode, "") if fits else ""
     occ = params.get("occupancy", 0)
     return "k_wavepool" + kind + (f"_o{occ}" if occ else "")
 
