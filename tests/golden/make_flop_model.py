@@ -7,8 +7,9 @@ configuration's seed on its subsampled pixel grid, and c_e the fixed FP32 cost
 of the event's arithmetic (FMA = 2, add/mul/cmp/min/max/cvt = 1, each
 transcendental or sqrt/div = 1).  pcg4d integer work is reported separately.
 
-Run from the repository root:  python tests/golden/make_flop_model.py
-Writes sky-tracer_amd/pysky/flop_model.json.
+Run from the repository root:  python tests/golden/make_flop_model.py [--missing]
+Writes sky-tracer_amd/pysky/flop_model.json (--missing: adds the configurations
+the committed file lacks, leaving its entries as they are).
 """
 import json
 import os
@@ -76,26 +77,35 @@ def measure(workload, mode, layers, growth=1.0):
     }
 
 
-def main():
+def main(missing_only=False):
+    """Measures every configuration; with missing_only (--missing) keeps the
+    committed entries as they are and measures only configurations they lack."""
+    dst = os.path.join(ROOT, "sky-tracer_amd", "pysky", "flop_model.json")
     out = {"costs": COSTS, "int_ops_per_rng_draw": INT_OPS_PER_DRAW, "configs": {}}
+    if missing_only:
+        with open(dst) as f:
+            out = json.load(f)
     for wl in ("c1", "c2", "c3"):
         for mode in ("reference", "carried", "carried_is", "carried_is_ratio"):
             if mode in ("carried_is", "carried_is_ratio"):
                 combos = [(L, 1.0) for L in (4, 5, 6, 7, 8, 12, 16, 24, 32, 64)]
-                combos += [(L, g) for L in (2, 3, 4, 5, 6, 7, 8) for g in (1.5, 2.0, 2.5, 3.0, 4.0, 5.0)]
+                combos += [(L, g) for L in (2, 3, 4, 5, 6, 7, 8) for g in (1.5, 2.0, 2.5, 3.0, 4.0, 5.0, 6.0)]
                 if mode == "carried_is_ratio":
                     combos = [(L, g) for L, g in combos if L <= 8]
+            elif mode == "reference":
+                combos = [(16, 1.0), (64, 1.0)] + [(L, g) for L in (2, 3) for g in (4.0, 5.0, 6.0)]
             else:
                 combos = [(16, 1.0), (64, 1.0)]
             for layers, g in combos:
                 k = key(wl, mode, layers, g)
+                if missing_only and k in out["configs"]:
+                    continue
                 out["configs"][k] = measure(wl, mode, layers, g)
                 print(k, round(out["configs"][k]["flops_per_sample"], 1), flush=True)
-    dst = os.path.join(ROOT, "sky-tracer_amd", "pysky", "flop_model.json")
     with open(dst, "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
     print("wrote", dst)
 
 
 if __name__ == "__main__":
-    main()
+    main(missing_only="--missing" in sys.argv[1:])
