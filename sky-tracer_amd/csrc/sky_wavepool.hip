@@ -1145,6 +1145,8 @@ SKY_WAVEPOOL_OCC3(8)
 // Path-replay adjoint: the forward state machine plus the score bookkeeping.
 __global__ __launch_bounds__(256) void k_wavepool_adjoint(PoolArgs a, AdjArgs A) { wavepool_body<true, 0>(a, A); }
 __global__ __launch_bounds__(256) void k_wavepool_adjoint_fast(PoolArgs a, AdjArgs A) { wavepool_body<true, 1>(a, A); }
+__global__ __launch_bounds__(256) void k_wavepool_adjoint_fastref(PoolArgs a, AdjArgs A) { wavepool_body<true, 2>(a, A); }
+__global__ __launch_bounds__(256) void k_wavepool_adjoint_fastrt(PoolArgs a, AdjArgs A) { wavepool_body<true, 3>(a, A); }
 
 // Per job (blockIdx.y): reduces the chunk partials in chunk order,
 // normalises by 1/spp and clamps (integrator.rs:307-310), and forms
@@ -1305,8 +1307,10 @@ hipError_t launch_adjoint(const DevJob *d_jobs, const WavePlan &p, uint32_t *met
     a.maj_lds = p.maj_lds;
     a.shell_lds = p.shell_lds;
     a.lam_lds = p.lam_lds;
-    hipLaunchKernelGGL(p.fast == 1 ? k_wavepool_adjoint_fast : k_wavepool_adjoint, dim3(p.grid), dim3(p.block),
-                       p.lds_bytes, stream, a, A);
+    typedef void (*AdjKernel)(PoolArgs, AdjArgs);
+    static const AdjKernel kernels[4] = {k_wavepool_adjoint, k_wavepool_adjoint_fast, k_wavepool_adjoint_fastref,
+                                         k_wavepool_adjoint_fastrt};
+    hipLaunchKernelGGL(kernels[p.fast], dim3(p.grid), dim3(p.block), p.lds_bytes, stream, a, A);
     return hipGetLastError();
 }
 
