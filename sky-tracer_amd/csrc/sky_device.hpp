@@ -68,6 +68,12 @@ struct DevScene {
     const uint16_t *tab_idx;   // [tab_nidx]
     uint32_t tab_k, tab_nidx;
     float tab_inv_dh;
+    // One-load interval lookup (null: use tab_idx): per cell of a uniform grid
+    // of tab_ncell cells, {k0, z[k0], inv[k0], za} {inv_a, zb, inv_b, 0}; the
+    // interval of h in cell j is k0 + (h >= za) + (h >= zb).
+    const float4 *tab_cell;    // [2 * tab_ncell]
+    uint32_t tab_ncell;
+    float tab_cell_inv_dh;
     const DevLambda *lam;
     const float *maj;  // [lambda * L + layer]
     const double *shell_d;  // [L + 1]: r_k^2 - R^2 = h_k (2R + h_k)
@@ -405,8 +411,22 @@ __device__ __forceinline__ uint32_t tab_interval(const DevScene &s, float h)
 // breakpoint grid makes every coefficient linear within an interval.
 __device__ __forceinline__ Coeffs tab_medium(const DevScene &s, uint32_t band, float h)
 {
-    const uint32_t k = tab_interval(s, h);
-    float t = (h - s.tab_z[k]) * s.tab_inv[k];
+    uint32_t k;
+    float z0, inv;
+    if (s.tab_cell) {  // one record load, then selects: the same k, z[k] and inv[k]
+        uint32_t j = (uint32_t)(h * s.tab_cell_inv_dh);
+        j = j < s.tab_ncell ? j : s.tab_ncell - 1;
+        const float4 c0 = s.tab_cell[2 * j], c1 = s.tab_cell[2 * j + 1];
+        const bool ua = h >= c0.w, ub = h >= c1.y;
+        k = __float_as_uint(c0.x) + (ua ? 1u : 0u) + (ub ? 1u : 0u);
+        z0 = ub ? c1.y : (ua ? c0.w : c0.y);
+        inv = ub ? c1.z : (ua ? c1.x : c0.z);
+    } else {
+        k = tab_interval(s, h);
+        z0 = s.tab_z[k];
+        inv = s.tab_inv[k];
+    }
+    float t = (h - z0) * inv;
     t = fminf(fmaxf(t, 0.0f), 1.0f);
     const float4 a = s.tab[band * s.tab_k + k], b = s.tab[band * s.tab_k + k + 1];
     return Coeffs{fmaf(b.x - a.x, t, a.x), fmaf(b.y - a.y, t, a.y), fmaf(b.z - a.z, t, a.z),

@@ -21,11 +21,18 @@ struct MediumTable {
     std::vector<float> sig, z, inv;
     std::vector<uint16_t> idx;
     float inv_dh = 0.0f;
+    // Interval records of a uniform cell grid, 8 floats per cell (empty when
+    // some cell would need more than two breakpoints): see build_cells.
+    std::vector<float> cell;
+    float cell_inv_dh = 0.0f;
     bool empty() const { return z.empty(); }
     size_t off_z() const { return (sig.size() * 4 + 15) & ~size_t(15); }
     size_t off_inv() const { return off_z() + ((z.size() * 4 + 15) & ~size_t(15)); }
     size_t off_idx() const { return off_inv() + ((inv.size() * 4 + 15) & ~size_t(15)); }
-    size_t bytes() const { return off_idx() + ((idx.size() * 2 + 15) & ~size_t(15)); }
+    // 64-byte aligned, so a 32-byte record never straddles a cache line; every
+    // blob size is a multiple of 64, so blobs packed back to back stay aligned.
+    size_t off_cell() const { return (off_idx() + idx.size() * 2 + 63) & ~size_t(63); }
+    size_t bytes() const { return (off_cell() + cell.size() * 4 + 63) & ~size_t(63); }
     void pack(std::vector<uint8_t> &blob) const;
     // Points the scene's tab* fields into a device copy of the blob.
     void attach(DevScene &s, const void *dev) const;

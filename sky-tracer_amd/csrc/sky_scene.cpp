@@ -110,6 +110,97 @@ static std::vector<double> table_breakpoints(const sky_medium_tables *t, double 
     return out;
 }
 
+// Interval of h among the FP32 breakpoints: the number of z[1..K-2] <= h,
+// which is what tab_interval's index-and-walk search returns.
+static uint32_t interval_by_count(const std::vector<float> &z, float h)
+{
+    return (uint32_t)(std::upper_bound(z.begin() + 1, z.end() - 1, h) - (z.begin() + 1));
+}
+
+// The kernel's one-load lookup (tab_medium) on the host.
+static uint32_t interval_by_cell(const MediumTable &m, float h, float &z0, float &inv)
+{
+    const uint32_t n = (uint32_t)(m.cell.size() / 8);
+    uint32_t j = (uint32_t)(h * m.cell_inv_dh);
+    j = j < n ? j : n - 1;
+    const float *c = &m.cell[(size_t)j * 8];
+    const bool ua = h >= c[3], ub = h >= c[5];
+    uint32_t k0;
+    std::memcpy(&k0, c, 4);
+    z0 = ub ? c[5] : (ua ? c[3] : c[1]);
+    inv = ub ? c[6] : (ua ? c[4] : c[2]);
+    return k0 + (ua ? 1u : 0u) + (ub ? 1u : 0u);
+}
+
+// Records for the kernel's one-load interval lookup.  An altitude h in
+// [0, top] goes to cell j = trunc(fl(h * fl(n / top))), clamped to n - 1.  The
+// two roundings put h * n / top within 2^-23 n (1.01) < 0.008 cells of the
+// computed product for n <= 65536, so every h sent to cell j lies strictly
+// within 0.01 cells of [j, j + 1) (all of [0, top] for the clamped last
+// cell).  Breakpoints at or below that range's start are counted into k0; the
+// ones inside it (at most two, else the next finer grid is tried) become
+// za <= zb, unused ones +inf.  The records are then checked against the
+// counting search at every breakpoint and cell boundary and their FP32
+// neighbours; with no grid of up to 65536 cells passing, the table keeps the
+// index-and-walk search alone.
+static void build_cells(MediumTable &m, double top)
+{
+    const std::vector<float> &z = m.z;
+    const size_t K = z.size();
+    m.cell.clear();
+    if (K < 2) return;
+    for (uint32_t n = 1024; n <= 65536; n *= 2) {
+        const double dh = top / n;
+        std::vector<float> rec((size_t)n * 8, 0.0f);
+        bool fits = true;
+        uint32_t k0 = 0;  // #{z[1..K-2] <= lo}, non-decreasing in j
+        for (uint32_t j = 0; j < n && fits; ++j) {
+            const double lo = j == 0 ? -INFINITY : (j - 0.01) * dh;
+            const double hi = j + 1 == n ? INFINITY : (j + 1.01) * dh;
+            while (k0 + 2 < K && (double)z[k0 + 1] <= lo) ++k0;
+            uint32_t ns = 0;
+            while (k0 + ns + 2 < K && (double)z[k0 + ns + 1] <= hi) ++ns;
+            if (ns > 2) {
+                fits = false;
+                break;
+            }
+            float *c = &rec[(size_t)j * 8];
+            std::memcpy(c, &k0, 4);
+            c[1] = z[k0];
+            c[2] = m.inv[k0];
+            c[3] = ns > 0 ? z[k0 + 1] : INFINITY;
+            c[4] = ns > 0 ? m.inv[k0 + 1] : 0.0f;
+            c[5] = ns > 1 ? z[k0 + 2] : INFINITY;
+            c[6] = ns > 1 ? m.inv[k0 + 2] : 0.0f;
+        }
+        if (!fits) continue;
+        m.cell.swap(rec);
+        m.cell_inv_dh = (float)(n / top);
+        const float topf = (float)top;
+        std::vector<float> probe{0.0f, topf};
+        for (size_t i = 0; i < K; ++i) probe.push_back(z[i]);
+        for (uint32_t j = 0; j <= n; ++j) probe.push_back((float)(j * dh));
+        bool ok = true;
+        for (float p : probe) {
+            float up = p, dn = p;
+            for (int s = 0; s <= 4 && ok; ++s) {
+                for (float q : {up, dn}) {
+                    if (!(q >= 0.0f && q <= topf)) continue;
+                    float z0, inv;
+                    const uint32_t k = interval_by_cell(m, q, z0, inv);
+                    const uint32_t kr = interval_by_count(z, q);
+                    if (k != kr || z0 != z[kr] || inv != m.inv[kr]) ok = false;
+                }
+                up = std::nextafter(up, INFINITY);
+                dn = std::nextafter(dn, -INFINITY);
+            }
+            if (!ok) break;
+        }
+        if (!ok) m.cell.clear();
+        return;
+    }
+}
+
 static void build_medium_table(const sky_atmosphere *a, const sky_wavelengths *wl, double top, MediumTable &m)
 {
     const sky_medium_tables *t = a->tables;
@@ -137,6 +228,7 @@ static void build_medium_table(const sky_atmosphere *a, const sky_wavelengths *w
         while (k + 2 < K && z[k + 1] <= x) ++k;
         m.idx[j] = (uint16_t)k;
     }
+    build_cells(m, top);
 }
 
 void MediumTable::pack(std::vector<uint8_t> &blob) const
@@ -146,6 +238,7 @@ void MediumTable::pack(std::vector<uint8_t> &blob) const
     std::memcpy(blob.data() + off_z(), z.data(), z.size() * 4);
     std::memcpy(blob.data() + off_inv(), inv.data(), inv.size() * 4);
     std::memcpy(blob.data() + off_idx(), idx.data(), idx.size() * 2);
+    if (!cell.empty()) std::memcpy(blob.data() + off_cell(), cell.data(), cell.size() * 4);
 }
 
 void MediumTable::attach(DevScene &s, const void *dev) const
@@ -162,6 +255,9 @@ void MediumTable::attach(DevScene &s, const void *dev) const
     s.tab_k = (uint32_t)z.size();
     s.tab_nidx = (uint32_t)idx.size();
     s.tab_inv_dh = inv_dh;
+    s.tab_cell = cell.empty() ? nullptr : (const float4 *)(p + off_cell());
+    s.tab_ncell = (uint32_t)(cell.size() / 8);
+    s.tab_cell_inv_dh = cell_inv_dh;
 }
 
 bool validate_tables(const sky_atmosphere *a, const sky_wavelengths *wl, std::string &err)
