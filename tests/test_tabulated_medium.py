@@ -11,9 +11,13 @@ Pins:
   * the trapezoid rule, exact for a function linear between breakpoints, for
     the vertical optical depth;
   * the ozone column of the committed AFGL profile in Dobson units, and the
-    sun radiance E / (2 pi (1 - cos r)) of data.rs:30-33.
+    sun radiance E / (2 pi (1 - cos r)) of data.rs:30-33;
+  * the counting search (number of breakpoints at or below h) for the
+    kernel's two interval lookups, on the library's own host-side tables.
 """
 import os
+import shutil
+import subprocess
 
 import numpy as np
 import pytest
@@ -141,3 +145,35 @@ def test_invalid_tables_are_rejected(ref):
     scene.atmosphere.tables = dataclasses.replace(tab, aerosol_optics=opt)
     with pytest.raises(InvalidConfig):
         pysky.wavelength_table(scene)
+
+
+def test_interval_lookups_match_the_counting_search(tmp_path):
+    """The kernel's one-load cell lookup and its index-and-walk search, run on
+    the host over tables built by the library's build_medium_table
+    (tests/cell_lookup_check.cpp), pick the interval the counting search picks,
+    with the same start and reciprocal width, at random altitudes and next to
+    every breakpoint and cell boundary.  Tables whose breakpoints sit on cell
+    boundaries, with FP32 n / top rounded either way, catch a cell range built
+    without its rounding margin (the host's own check then drops the cells,
+    which the cell counts catch)."""
+    hipcc = os.environ.get("HIPCC") or shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not found")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path / "cell_lookup_check")
+    subprocess.run([hipcc, "-O2", "-std=c++17", "-w", "-o", exe,
+                    os.path.join(root, "tests", "cell_lookup_check.cpp")], check=True)
+    out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
+    rows = {}
+    for line in out.strip().splitlines():
+        name, *fields = line.split()
+        rows[name] = {k: int(v) for k, v in (f.split("=") for f in fields)}
+    expect_cells = {"reference": 1024, "single_point": 1024, "cell_boundaries_top_100": 1024,
+                    "cell_boundaries_top_120": 1024, "fine_25m": 2048, "dense_cluster": 0,
+                    "irregular_top_120": 1024}
+    assert set(rows) == set(expect_cells)
+    for name, r in rows.items():
+        assert r["bad"] == 0, (name, r)
+        assert r["cells"] == expect_cells[name], (name, r)
+        assert r["tested"] > 300000, (name, r)
+    assert rows["reference"]["K"] == 50
