@@ -56,6 +56,31 @@ def test_tabulated_growing_shells_pair(gpu_context):
     assert frac >= 0.995
 
 
+def test_dense_breakpoints_keep_the_index_search(gpu_context):
+    """Three extra atmosphere rows 0.1 m apart just above 10 km, on the line
+    between the 10 and 11 km rows (so the profile is the same function), put
+    four breakpoints in one cell of even the finest (65536-cell) grid; the
+    library then keeps the index-and-walk interval search in place of the
+    one-load cell lookup, and its paths pair with the oracle the same way."""
+    import dataclasses
+    scene, cfg, op = ref_case(mode="carried_is")
+    tab = scene.atmosphere.tables
+    atm = tab.atmosphere
+    i = int(np.flatnonzero(atm[:, 0] == 10.0)[0])
+    lo, hi = atm[i], atm[i + 1]
+    extra = np.array([lo + (hi - lo) * (dz / (hi[0] - lo[0])) for dz in (1e-4, 2e-4, 3e-4)])
+    dense = dataclasses.replace(tab, atmosphere=np.vstack([atm[:i + 1], extra, atm[i + 1:]]))
+    scene.atmosphere.tables = dense
+    op = dataclasses.replace(op, tables=dense)
+    print("[gpu] tabulated, dense breakpoints (index search)", flush=True)
+    film = pysky.render(scene, cfg, context=gpu_context).values()
+    ref, _, _, _, rc = O.render(op, "f32")
+    assert rc == 0
+    frac, bad = paired_stats(film, ref.ravel(), rtol=1e-3)
+    print(f"paired fraction {frac:.5f} ({len(bad)} of {film.size} differ)")
+    assert frac >= 0.995
+
+
 def test_tabulated_optical_depth_matches_oracle(gpu_context):
     scene, _, op = ref_case()
     rng = np.random.default_rng(5)
