@@ -1143,10 +1143,19 @@ SKY_WAVEPOOL_OCC3(8)
 #undef SKY_WAVEPOOL_OCC
 
 // Path-replay adjoint: the forward state machine plus the score bookkeeping.
-__global__ __launch_bounds__(256) void k_wavepool_adjoint(PoolArgs a, AdjArgs A) { wavepool_body<true, 0>(a, A); }
-__global__ __launch_bounds__(256) void k_wavepool_adjoint_fast(PoolArgs a, AdjArgs A) { wavepool_body<true, 1>(a, A); }
-__global__ __launch_bounds__(256) void k_wavepool_adjoint_fastref(PoolArgs a, AdjArgs A) { wavepool_body<true, 2>(a, A); }
-__global__ __launch_bounds__(256) void k_wavepool_adjoint_fastrt(PoolArgs a, AdjArgs A) { wavepool_body<true, 3>(a, A); }
+// Capped at 4 waves per SIMD (128 VGPRs, a few spills; uncapped the replays
+// take 137-144 VGPRs and run 3 waves per SIMD): 7% less replay time at
+// 256x128 px x 32 bands.  A 5-wave cap (96 VGPRs) spills heavily and is slower.
+#define SKY_ADJOINT(NAME, FK)                                                                                \
+    __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void NAME(PoolArgs a, AdjArgs A) \
+    {                                                                                                        \
+        wavepool_body<true, FK>(a, A);                                                                       \
+    }
+SKY_ADJOINT(k_wavepool_adjoint, 0)
+SKY_ADJOINT(k_wavepool_adjoint_fast, 1)
+SKY_ADJOINT(k_wavepool_adjoint_fastref, 2)
+SKY_ADJOINT(k_wavepool_adjoint_fastrt, 3)
+#undef SKY_ADJOINT
 
 // Per job (blockIdx.y): reduces the chunk partials in chunk order,
 // normalises by 1/spp and clamps (integrator.rs:307-310), and forms
