@@ -156,7 +156,9 @@ typedef struct sky_tuning {
     uint32_t pixels_per_item; /* pixels per wave work item (128 / count)       */
     uint32_t event_threshold; /* lanes waiting before events are handled (8)   */
     uint32_t track_steps;     /* tracking micro-steps between handler passes (2) */
-    uint32_t occupancy;       /* 0: compiler's register budget; 4, 5, 6, 8: at least that many waves per SIMD */
+    uint32_t occupancy;       /* forward renders: 0 the compiler's register budget; 4, 5, 6, 8 at
+                                 least that many waves per SIMD (the adjoint's replay kernels are
+                                 always built for 4) */
 } sky_tuning;
 
 /* ---- context ---- */
