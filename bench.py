@@ -533,11 +533,14 @@ def main():
             args.shell_growth = p.get("shell_growth", 1.0) if args.shell_growth <= 0 else args.shell_growth
             out = bench_batch(args, ctx)
         else:
-            # the adjoint and the fit use the large-render tuning (shells and launch
-            # shape; a sweep of the C5 forward + replay finds the same launch shape)
+            # the adjoint and the fit use the large-render tuning (shells, threshold,
+            # steps, occupancy) with 40 waves per CU: with the replay kernels at 4
+            # waves per SIMD, forward + replay at 256x128 x 32 bands take 57.7 ms
+            # at 40 waves per CU, 58.3 at 32 and 60.0 at 24
             p = autotune.tuned(pysky.SceneData(wavelengths_nm=list(pysky.midpoint_grid(64))),
                                pysky.RenderConfig(width=1024, height=512, carried_tau=True,
                                                   phase_importance_sampling=True))
+            p = dict(p, waves_per_cu=40)
             autotune.apply(ctx, pysky.RenderConfig(), p)
             args.layers = args.layers or p["majorant_layers"]
             args.shell_growth = p.get("shell_growth", 1.0) if args.shell_growth <= 0 else args.shell_growth
