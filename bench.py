@@ -33,6 +33,9 @@ MODES = {
     "carried_is": dict(carried_tau=True, phase_importance_sampling=True),
     "carried_is_ratio": dict(carried_tau=True, phase_importance_sampling=True, ratio_tracking=True),
 }
+# how oracle/Makefile builds the CPU baseline (gcc -O3 -march=x86-64-v3 measured no faster on
+# this path: 10.2 against 9.9 M samples/s on 8 threads)
+ORACLE_BUILD = "gcc -O2 -std=c11 -fopenmp (scalar, no -march), OpenMP dynamic schedule over pixels"
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 (packed vector FMA); unpacked vector issue is half
 # HBM bytes per integrator launch from rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (scales from
 # a 256 MiB copy): profiles/k_wavepool_c2_1024x512x64_counters.md.  The launch writes one
@@ -78,10 +81,12 @@ def make_config(w, mode, layers, seed=1):
                               **MODES[mode])
 
 
-def cpu_baseline(wname, w, mode, layers, target_s=15.0):
+def cpu_baseline(wname, w, mode, layers, shell_growth, target_s=15.0):
     """The oracle (C restatement of the same estimator, FP32) on the host
     cores: every sample of the workload for an evenly spread subset of its
-    pixels, the subset sized from a short probe so the run takes ~target_s."""
+    pixels, the subset sized from a short probe so the run takes ~target_s.
+    It traces the GPU run's paths: same estimator flags, majorant shells
+    (count and thickness growth), depth cap and roulette."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
 
@@ -97,7 +102,8 @@ def cpu_baseline(wname, w, mode, layers, target_s=15.0):
         p = O.Params(width=w["width"], height=w["height"], spp=spp, seed=1, sun_elevation_deg=w["sun"],
                      max_depth=w["max_depth"], ground_albedo=w["albedo"],
                      aerosol_sca_550_m_inv=3.996e-6 * w["aerosol"], aerosol_abs_550_m_inv=4.4e-6 * w["aerosol"],
-                     lambda_nm=list(O.midpoint_grid(w["n_lambda"])), flags=flags, majorant_layers=layers)
+                     lambda_nm=list(O.midpoint_grid(w["n_lambda"])), flags=flags, majorant_layers=layers,
+                     shell_growth=shell_growth, rr_start_depth=4)
         t0 = time.perf_counter()
         O.render_pixels(p, pixels, "f32", threads=threads)
         return time.perf_counter() - t0, len(pixels) * w["n_lambda"] * spp
@@ -114,7 +120,9 @@ def cpu_baseline(wname, w, mode, layers, target_s=15.0):
         "cores": threads,
         "kind": "port",
         "sample": f"oracle FP32, {len(pixels)} of {w['width']}x{w['height']} px (evenly spread) x "
-                  f"{w['n_lambda']} lambda x {w['spp']} spp, {dt:.1f} s on {threads} threads",
+                  f"{w['n_lambda']} lambda x {w['spp']} spp, {layers} shells growing {shell_growth:g}x, "
+                  f"{dt:.1f} s on {threads} threads",
+        "build": ORACLE_BUILD,
     }
 
 
@@ -606,7 +614,7 @@ def main():
                         "kernel_ms_avg": round(avg_kernel_s * 1e3, 2)}
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(args.workload, w, args.mode, args.layers)
+        cpu = cpu_baseline(args.workload, w, args.mode, args.layers, cfg.shell_growth)
     out = {
         "metric": "spectral path samples/s (path x lambda) on 1 MI355X",
         "value": value,

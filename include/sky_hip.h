@@ -129,7 +129,10 @@ typedef struct sky_render_config {
     double majorant_margin;        /* 1.05 in the reference            */
     /* Optional host table [count * majorant_layers] (km^-1) replacing the
      * computed majorant grid, e.g. to hold it fixed while parameters move
-     * (finite differences with common random numbers, the adjoint). */
+     * (finite differences with common random numbers, the adjoint).  Every
+     * entry must bound its shell's extinction (sky_majorant_grid with
+     * margin 1 is the smallest valid table); a smaller one is rejected with
+     * SKY_ERR_INVALID_CONFIG, since delta tracking would be biased. */
     const float *majorant_table;
     /* Majorant shell thickness growth from the ground up: 1 (or 0) = the
      * reference's uniform shells; g > 1 gives h_k = top (g^k-1)/(g^L-1),

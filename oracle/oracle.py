@@ -142,6 +142,7 @@ def lib():
         L.orc_optical_depth.restype = C.c_double
         L.orc_optical_depth_band.argtypes = [P, C.c_uint32, dp, dp, C.c_double]
         L.orc_optical_depth_band.restype = C.c_double
+        L.orc_optical_depth_batch.argtypes = [P, C.c_uint32, C.c_uint32, dp, dp, dp, dp]
         L.orc_coefficients_band.argtypes = [P, C.c_uint32, C.c_double, dp, dp, dp, dp]
         L.orc_camera_dir.argtypes = [P, C.c_double, C.c_double, dp]
         for s in ("f32", "f64"):
@@ -380,6 +381,18 @@ def optical_depth(p: Params, band: int, o, d, t_max: float) -> float:
     o = np.ascontiguousarray(o, dtype=np.float64)
     d = np.ascontiguousarray(d, dtype=np.float64)
     return float(lib().orc_optical_depth_band(C.byref(cp), band, _dp(o), _dp(d), t_max))
+
+
+def optical_depth_batch(p: Params, band: int, o, d, t_max) -> np.ndarray:
+    """optical_depth for many rays: o, d [n, 3], t_max [n] -> tau [n]."""
+    cp = p.to_c()
+    o = np.ascontiguousarray(np.reshape(o, (-1, 3)), dtype=np.float64)
+    d = np.ascontiguousarray(np.reshape(d, (-1, 3)), dtype=np.float64)
+    t = np.ascontiguousarray(np.ravel(t_max), dtype=np.float64)
+    out = np.zeros(t.size)
+    if lib().orc_optical_depth_batch(C.byref(cp), band, t.size, _dp(o), _dp(d), _dp(t), _dp(out)):
+        raise ValueError("invalid band or parameters")
+    return out
 
 
 def camera_dir(p: Params, u: float, v: float) -> np.ndarray:

@@ -581,6 +581,22 @@ double orc_optical_depth_band(const orc_params *p, uint32_t band, const double o
     return orc_optical_depth(p, &lam[band], o, d, t_max);
 }
 
+/* orc_optical_depth_band over n rays (origins and directions [n][3], lengths
+ * [n]) in one call, for quadratures that need many transmittances. */
+int orc_optical_depth_batch(const orc_params *p, uint32_t band, uint32_t n, const double *o, const double *d,
+                            const double *t_max, double *out)
+{
+    orc_lambda lam[ORC_MAX_LAMBDA];
+    if (band >= p->n_lambda || orc_lambda_table(p, lam)) return 1;
+    gl16_init();
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        od_band = band;
+        out[i] = orc_optical_depth(p, &lam[band], o + 3 * i, d + 3 * i, t_max[i]);
+    }
+    return 0;
+}
+
 double orc_optical_depth(const orc_params *p, const orc_lambda *lam, const double o[3],
                          const double d[3], double t_max)
 {

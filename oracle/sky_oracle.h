@@ -154,6 +154,9 @@ double orc_optical_depth_band(const orc_params *p, uint32_t band, const double o
                               double t_max);
 double orc_optical_depth(const orc_params *p, const orc_lambda *lam, const double o[3],
                          const double d[3], double t_max);
+/* orc_optical_depth_band for n rays: o, d [n][3], t_max [n] -> out [n]; 0 on success. */
+int orc_optical_depth_batch(const orc_params *p, uint32_t band, uint32_t n, const double *o, const double *d,
+                            const double *t_max, double *out);
 
 /* Camera (sky_trace.wgsl:610-689). */
 void orc_camera_dir(const orc_params *p, double u, double v, double dir[3]);

@@ -495,6 +495,22 @@ sky_status build_scene(const sky_planet *planet, const sky_atmosphere *a, const 
                 return SKY_ERR_INVALID_CONFIG;
             }
         }
+        // Delta tracking samples the free-flight law only where the majorant
+        // bounds sigma_t; below it the accept ratio would be clamped to 1 and
+        // the render silently biased.  Each entry must reach its shell's
+        // exact maximum extinction (the reference's 1e-8 floor included),
+        // up to the FP32 rounding of a table made with margin 1.
+        std::vector<float> exact;
+        fill_majorants(planet, a, wl, cfg->majorant_layers, 1.0, growth, exact);
+        for (size_t i = 0; i < n; ++i) {
+            if (out.maj[i] < exact[i] * (1.0f - 1e-6f)) {
+                err = "majorant table entry " + std::to_string(i) + " (band " +
+                      std::to_string(i / cfg->majorant_layers) + ", shell " +
+                      std::to_string(i % cfg->majorant_layers) + ") is below the shell's maximum extinction " +
+                      std::to_string(exact[i]) + " km^-1";
+                return SKY_ERR_INVALID_CONFIG;
+            }
+        }
     } else {
         fill_majorants(planet, a, wl, cfg->majorant_layers, cfg->majorant_margin, growth, out.maj);
     }

@@ -9,13 +9,15 @@ shadow rays, 16 majorant shells).  Written: c2_fp64_reference.npz with the
 XYZ means of the grid pixels and their standard errors (from per-wavelength
 second moments; wavelengths are independent streams).
 
-Resumable: progress (double sums per sample chunk) is kept in
-_work/c2_reference_state.npz.  Run from the repository root:
-    python tests/golden/make_c2_reference.py [--threads 6] [--spp 65536]
+Resumable: progress (double sums per sample chunk) is kept in a state file
+(--state, default c2_reference_state.npz in the system temp directory).
+Run from the repository root:
+    python tests/golden/make_c2_reference.py [--threads 6] [--spp 65536] [--state PATH]
 """
 import argparse
 import os
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
@@ -44,8 +46,10 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--threads", type=int, default=6)
     ap.add_argument("--spp", type=int, default=65536)
+    ap.add_argument("--state", default=os.path.join(tempfile.gettempdir(), "c2_reference_state.npz"),
+                    help="resumable progress file")
     args = ap.parse_args()
-    state_path = os.path.join(ROOT, "_work", "c2_reference_state.npz")
+    state_path = args.state
     pix = grid_pixels()
     if os.path.exists(state_path):
         st = np.load(state_path)

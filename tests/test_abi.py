@@ -105,6 +105,26 @@ def test_invalid_atmosphere(field, value):
         pysky.render(scene, pysky.RenderConfig(width=4, height=2, spp=1), context=_null_context())
 
 
+@pytest.mark.parametrize("growth", [1.0, 4.0])
+def test_majorant_table_must_bound_the_extinction(growth):
+    """A caller's fixed majorant below some shell's maximum extinction would
+    bias delta tracking (the accept ratio is clamped at 1), so it is refused
+    before the device; the margin-1 grid itself is the smallest valid one."""
+    scene = pysky.load_scene(10.0)
+    cfg = pysky.RenderConfig(width=4, height=2, spp=1, majorant_layers=3, shell_growth=growth)
+    exact = pysky.majorant_grid(scene, 3, 1.0, growth)
+    for table in (exact, pysky.majorant_grid(scene, 3, 1.25, growth)):
+        cfg.majorant_table = table
+        with pytest.raises(pysky.NoAdapter):
+            pysky.render(scene, cfg, context=_null_context())
+    for band, shell in ((0, 0), (scene.band_count - 1, 2)):
+        low = np.array(exact, dtype=np.float32).reshape(scene.band_count, 3).copy()
+        low[band, shell] *= 0.999
+        cfg.majorant_table = low
+        with pytest.raises(pysky.InvalidConfig, match="below the shell's maximum extinction"):
+            pysky.render(scene, cfg, context=_null_context())
+
+
 class _NullCtx:
     handle = None
 

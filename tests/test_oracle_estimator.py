@@ -198,3 +198,207 @@ def test_oracle_watchdog_and_invalid():
     p = O.Params(width=0)
     _, _, _, _, rc = O.render(p, "f32")
     assert rc == 1
+
+
+# ------------------------------------------------------- Russian roulette
+
+def _independent(seed_lo, k):
+    """A seed whose sample keys (sample ^ seed_hi) share no stream with
+    seed_hi = 0 for spp below 65536."""
+    return (k * 65536 << 32) | seed_lo
+
+
+@pytest.mark.parametrize("flags", [O.FLAG_CARRIED_TAU | O.FLAG_PHASE_IS, 0], ids=["lobe", "reference"])
+@pytest.mark.parametrize("rr_start,spp", [(0, 64), (4, 512)])
+def test_russian_roulette_is_unbiased(flags, rr_start, spp):
+    """russian_roulette (sky_trace.wgsl:691-701) keeps the expectation: a
+    render with roulette from rr_start agrees with one where it never fires
+    (rr_start = max_depth), on independent streams.  The scene makes deep
+    paths matter (albedo 0.8, aerosol x4, depth <= 12).  Dropping the 1/q
+    weight moves the image by -13% (rr_start 0) and -1% (rr_start 4), i.e.
+    z ~ -50 and ~ -8 at these sample counts."""
+    base = dict(width=64, height=32, spp=spp, sun_elevation_deg=10.0, max_depth=12, ground_albedo=0.8,
+                aerosol_sca_550_m_inv=4 * 3.996e-6, aerosol_abs_550_m_inv=4 * 4.4e-6,
+                lambda_nm=list(O.midpoint_grid(8)), flags=flags, majorant_layers=8)
+    fa, ma, _, ca, ra = O.render(O.Params(rr_start_depth=rr_start, seed=1, **base), "f32", threads=8, want_m2=True)
+    fb, mb, _, cb, rb = O.render(O.Params(rr_start_depth=12, seed=_independent(1, 1), **base), "f32", threads=8,
+                                 want_m2=True)
+    assert ra == 0 and rb == 0
+    # roulette did end paths early
+    assert ca["scatter_vertices"] < 0.97 * cb["scatter_vertices"]
+    va = np.maximum(ma - fa ** 2, 0) / spp
+    vb = np.maximum(mb - fb ** 2, 0) / spp
+    sun = (fa.max(axis=2, keepdims=True) < 50) & (fb.max(axis=2, keepdims=True) < 50)
+    mask = np.broadcast_to(sun, fa.shape)
+    z_tot = (fa[mask].sum() - fb[mask].sum()) / math.sqrt(va[mask].sum() + vb[mask].sum())
+    z = (fa - fb)[mask] / np.sqrt(va[mask] + vb[mask] + 1e-300)
+    assert abs(z_tot) < 4.0
+    assert abs(z.mean()) < 0.06 and 0.9 < z.std() < 1.1
+
+
+# ------------------------------------- two bounces: the ground's sky light
+# A ground pixel at max_depth 2 (sky-view LUT, fixed pixel-centre rays) sees
+#   L = L_ss(view) + T_view [ground NEE + albedo/pi * int_up cos L_ss dw] + L_c
+# (sky_trace.wgsl:750-766 then :733-744 one vertex later): the view ray's own
+# single scattering, the ground's direct sun, the Lambertian bounce into the
+# sky whose next vertex adds its single-scattered sun, and the view segment's
+# second-order term L_c.  The observer stands 10 m up, so L_c is ~5e-4 of L
+# and is taken to first order in the segment's optical depth.  Every optical
+# depth comes from the pinned quadrature (orc_optical_depth, mpmath-checked).
+
+def _gl(n):
+    return np.polynomial.legendre.leggauss(n)
+
+
+def _sigma(p, row, h):
+    br, bo, bas, _ = row[:4]
+    inside = (h >= 0) & (h <= p.atmosphere_radius_km - p.ground_radius_km)
+    return (np.where(inside, br * np.exp(-h / p.rayleigh_h_km), 0.0),
+            np.where(inside, bas * np.exp(-h / p.aerosol_h_km), 0.0))
+
+
+def _boundary(p, o, d, eps=0.005):
+    """next_boundary (sky_trace.wgsl:233-250) for rays [n, 3]: (t, kind)."""
+    R, Rt = p.ground_radius_km, p.atmosphere_radius_km
+    b = np.sum(o * d, axis=1)
+    oo = np.sum(o * o, axis=1)
+    disc_t = b * b - (oo - Rt * Rt)
+    t_exit = -b + np.sqrt(np.maximum(disc_t, 0))
+    disc_g = b * b - (oo - R * R)
+    sg = np.sqrt(np.maximum(disc_g, 0))
+    tg = np.where(-b - sg > eps, -b - sg, np.where(-b + sg > eps, -b + sg, -1.0))
+    ground = (disc_g >= 0) & (tg > eps) & (tg < t_exit)
+    return np.where(ground, tg, t_exit), np.where(disc_t >= 0, np.where(ground, 0, 1), -1)
+
+
+def _nodes(p, o, d, tmax, n=8):
+    """Composite Gauss-Legendre nodes on [0, tmax], split where the ray
+    crosses a ladder of altitudes, the ozone kinks and the perigee."""
+    R = p.ground_radius_km
+    b, oo = float(np.dot(o, d)), float(np.dot(o, o))
+    hs = [0.05, 0.1, 0.2, 0.4, 0.8, 1.2, 1.8, 2.5, 3.5, 5, 7, 10, 13, 16, 20, 30, 40, 50, 60, 75, 90,
+          p.ozone_center_km - p.ozone_half_width_km, p.ozone_center_km, p.ozone_center_km + p.ozone_half_width_km]
+    cuts = [0.0, tmax] + ([-b] if 0 < -b < tmax else [])
+    for h in hs:
+        disc = b * b - (oo - (R + h) ** 2)
+        if disc >= 0:
+            cuts += [t for t in (-b - math.sqrt(disc), -b + math.sqrt(disc)) if 0 < t < tmax]
+    cuts = np.unique(cuts)
+    pts = [cuts[0]]
+    for a, e in zip(cuts[:-1], cuts[1:]):
+        m = int(math.ceil((e - a) / 20.0))
+        pts += list(a + (e - a) * np.arange(1, m + 1) / m)
+    pts = np.array(pts)
+    x, w = _gl(n)
+    a, e = pts[:-1, None], pts[1:, None]
+    return (0.5 * (a + e) + 0.5 * (e - a) * x).ravel(), (0.5 * (e - a) * w).ravel()
+
+
+def _phases(p, mu):
+    g = p.aerosol_g
+    return 3.0 / (16 * math.pi) * (1 + mu * mu), (1 - g * g) / (4 * math.pi * (1 + g * g - 2 * g * mu) ** 1.5)
+
+
+def _single_scatter_rays(p, band, origins, dirs):
+    """Single-scattered sun radiance along each ray (no surface term)."""
+    row = O.lambda_table(p)[band]
+    sun = O.sun_direction(p)
+    omega = 2 * math.pi * (1 - math.cos(p.sun_angular_radius_rad))
+    tb, kb = _boundary(p, origins, dirs)
+    ts, ws, idx = [], [], []
+    for i in np.where(kb >= 0)[0]:
+        t, w = _nodes(p, origins[i], dirs[i], tb[i])
+        ts.append(t), ws.append(w), idx.append(np.full(t.size, i))
+    t, w, idx = np.concatenate(ts), np.concatenate(ws), np.concatenate(idx)
+    o, d = origins[idx], dirs[idx]
+    x = o + t[:, None] * d
+    h = np.linalg.norm(x, axis=1) - p.ground_radius_km
+    tau_v = O.optical_depth_batch(p, band, o, d, t)
+    xs = x + 0.005 * sun
+    sd = np.tile(sun, (t.size, 1))
+    t_s, k_s = _boundary(p, xs, sd)
+    tau_s = O.optical_depth_batch(p, band, xs, sd, np.where(k_s == 1, t_s, 0.0))
+    sr, sas = _sigma(p, row, h)
+    pr, ph = _phases(p, d @ sun)
+    f = np.exp(-tau_v - tau_s) * (sr * pr + sas * ph) * row[4] * omega * (k_s == 1)
+    out = np.zeros(len(dirs))
+    np.add.at(out, idx, f * w)
+    return out
+
+
+def _ground_pixel_depth2(p, band, d, n_ang=12):
+    R = p.ground_radius_km
+    row = O.lambda_table(p)[band]
+    sun = O.sun_direction(p)
+    omega = 2 * math.pi * (1 - math.cos(p.sun_angular_radius_rad))
+    o = np.array([0.0, R + p.observer_altitude_km, 0.0])
+    tg, kg = _boundary(p, o[None], d[None])
+    assert kg[0] == 0
+    tg = float(tg[0])
+    xg = o + tg * d
+    nrm = xg / np.linalg.norm(xg)
+    L_view = _single_scatter_rays(p, band, o[None], d[None])[0]
+    T_view = math.exp(-O.optical_depth_batch(p, band, o[None], d[None], [tg])[0])
+    xs = xg + 0.005 * sun
+    t_s, k_s = _boundary(p, xs[None], sun[None])
+    T_sun = math.exp(-O.optical_depth_batch(p, band, xs[None], sun[None], t_s)[0]) if k_s[0] == 1 else 0.0
+    L_direct = p.ground_albedo / math.pi * row[4] * omega * max(float(nrm @ sun), 0.0) * T_sun
+    # the sky over the ground point: cos-weighted hemisphere, symmetric about the sun's plane
+    xm, wm = _gl(n_ang)
+    xp, wp = _gl(n_ang)
+    M, P = np.meshgrid(0.5 * (xm + 1), 0.5 * math.pi * (xp + 1), indexing="ij")
+    W = (np.outer(0.5 * wm, 0.5 * math.pi * wp) * 2.0).ravel()
+    M, P = M.ravel(), P.ravel()
+    sh = sun - (sun @ nrm) * nrm
+    e1 = sh / np.linalg.norm(sh)
+    e2 = np.cross(nrm, e1)
+    st = np.sqrt(1 - M * M)
+    tang = (st * np.cos(P))[:, None] * e1 + (st * np.sin(P))[:, None] * e2
+    up = M[:, None] * nrm + tang
+    L_up = _single_scatter_rays(p, band, xg + 0.005 * up, up)
+    E_sky = float(np.sum(W * M * L_up))
+    # second order on the view segment: phase-weighted incoming radiance at the ground point
+    pr_u, ph_u = _phases(p, up @ d)
+    pr_d, ph_d = _phases(p, (tang - M[:, None] * nrm) @ d)
+    I_R = np.sum(W * pr_u * L_up) + L_direct * np.sum(W * pr_d)
+    I_A = np.sum(W * ph_u * L_up) + L_direct * np.sum(W * ph_d)
+    x16, w16 = _gl(16)
+    t = 0.5 * tg * (x16 + 1)
+    tau = O.optical_depth_batch(p, band, np.tile(o, (16, 1)), np.tile(d, (16, 1)), t)
+    sr, sas = _sigma(p, row, np.linalg.norm(o + t[:, None] * d, axis=1) - R)
+    L_c = float(np.sum(0.5 * tg * w16 * np.exp(-tau) * (sr * I_R + sas * I_A)))
+    L_bounce = T_view * p.ground_albedo / math.pi * E_sky
+    return L_view + T_view * L_direct + L_bounce + L_c, L_bounce
+
+
+def test_fast_single_scatter_matches_adaptive_quadrature():
+    """The composite rule behind the two-bounce check against scipy's
+    adaptive one (_single_scatter) on three view directions."""
+    p = O.Params(lambda_nm=[450.0], sun_elevation_deg=10.0)
+    o = _obs(p.observer_altitude_km)
+    for d in ([0.3, 0.5, 0.8], [1, 0.05, 0.1], [0.2, -0.3, 0.9]):
+        d = np.array(d, dtype=float) / np.linalg.norm(d)
+        assert _single_scatter_rays(p, 0, o[None], d[None])[0] == pytest.approx(_single_scatter(p, 0, d), rel=1e-9)
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("flags", [O.FLAG_CARRIED_TAU | O.FLAG_PHASE_IS, 0], ids=["lobe", "reference"])
+def test_ground_bounce_into_sky_matches_quadrature(flags):
+    """Depth 2 over a bright ground (albedo 0.8, sun 20 deg): the FP64
+    oracle's ground pixels against the quadrature above, within 4 sigma.
+    The sky-bounce term is 26% (450 nm) and 8% (650 nm) of the pixel;
+    dropping the albedo from the bounce weight (wgsl:760) gives z ~ 35."""
+    spp = 1 << 18
+    W, H = 3, 9  # rows 7 (22.5 deg below the horizon) and 8 (nadir)
+    p = O.Params(width=W, height=H, spp=spp, projection=O.PROJ_SKY_VIEW_LUT, max_depth=2, ground_albedo=0.8,
+                 sun_elevation_deg=20.0, observer_altitude_km=0.01, lambda_nm=[450.0, 650.0], flags=flags)
+    pix = [(0, 7), (2, 7), (1, 8)]
+    film, m2, _, rc = O.render_pixels(p, [y * W + x for x, y in pix], "f64", threads=8)
+    assert rc == 0
+    for i, (x, y) in enumerate(pix):
+        d = O.camera_dir(p, (x + 0.5) / W, (y + 0.5) / H)
+        for b in range(2):
+            ref, bounce = _ground_pixel_depth2(p, b, d)
+            assert bounce > 0.07 * ref
+            sd = math.sqrt(max(m2[i, b] - film[i, b] ** 2, 0.0) / spp)
+            assert abs(film[i, b] - ref) < 4 * sd + 1e-5 * ref, ((x, y, b), film[i, b], ref, sd)

@@ -322,3 +322,69 @@ def test_panorama_camera_closed_form():
     assert np.allclose(O.camera_dir(p, 0.5, 0.5), [0, 0, 1], atol=1e-15)
     assert np.allclose(O.camera_dir(p, 0.25, 0.0), [0, 1, 0], atol=1e-12)
     assert np.allclose(O.camera_dir(p, 0.75, 0.5), [1, 0, 0], atol=1e-12)
+
+
+@pytest.mark.parametrize("h_obs", [0.2, 5.0])
+@pytest.mark.parametrize("azimuth", [0.0, 37.0, 200.0])
+def test_sky_view_lut_camera_closed_form(h_obs, azimuth):
+    """sky_view_lut_dir (sky_trace.wgsl:625-681) pinned by its geometry.
+    Pixel centres map to uv = (x/(W-1), y/(H-1)); with H = 9 the rows are
+    v = 0, 1/8, ..., 1:
+      * row 0 looks at the zenith and the last row at the nadir;
+      * row 6 (v = 0.75, the sky/ground split) is the geometric horizon: the
+        ray is tangent to the ground sphere (impact parameter R);
+      * rows above it see the sky (the ray leaves the atmosphere), rows below
+        hit the ground, and the zenith angle grows down the rows;
+      * column u = 0 faces the sun's azimuth, u = 1 faces away, and
+        u = 1/sqrt(2) (cos = 1 - 2u^2 = 0) faces side = up x forward;
+      * the non-linear parametrisation: zenith angle (pi - beta)(1 - (1 - v/0.75)^2)
+        in the sky, pi - beta + beta ((v - 0.75)/0.25)^2 below, azimuth
+        acos(1 - 2 u^2) from the sun."""
+    W, H = 3, 9
+    p = O.Params(width=W, height=H, projection=O.PROJ_SKY_VIEW_LUT, observer_altitude_km=h_obs,
+                 sun_azimuth_deg=azimuth, sun_elevation_deg=10.0)
+    R = p.ground_radius_km
+    vh = R + h_obs
+    o = np.array([0.0, vh, 0.0])
+    a = math.radians(azimuth)
+    fw = np.array([math.sin(a), 0.0, math.cos(a)])
+    side = np.array([math.cos(a), 0.0, -math.sin(a)])  # (0,1,0) x fw
+
+    def d(x, y, u=None):
+        return O.camera_dir(p, (x + 0.5) / W if u is None else u, (y + 0.5) / H)
+
+    for x in range(W):
+        assert np.allclose(d(x, 0), [0, 1, 0], atol=1e-12)
+        assert np.allclose(d(x, H - 1), [0, -1, 0], atol=1e-12)
+        hz = d(x, 6)
+        assert np.linalg.norm(np.cross(o, hz)) == pytest.approx(R, rel=1e-12)
+        assert hz[1] == pytest.approx(-math.sqrt(vh * vh - R * R) / vh, rel=1e-12)
+    zen = [math.acos(np.clip(d(1, y)[1], -1, 1)) for y in range(H)]
+    assert all(z1 < z2 for z1, z2 in zip(zen[:-1], zen[1:]))
+    for y in (1, 2, 3, 4, 5, 7):  # row 6 is tangent: checked above
+        hit, t, kind = O.next_boundary_reference(p, o, d(1, y))
+        assert hit and kind == (1 if y < 6 else 0), (y, kind)
+    # a dense sweep of v: sky exactly above the 0.75 split, ground below it
+    for vv in np.linspace(0.005, 0.995, 199):
+        if abs(vv - 0.75) < 1e-3:
+            continue
+        hit, t, kind = O.next_boundary_reference(p, o, O.camera_dir(p, 0.5, vv * (H - 1) / H + 0.5 / H))
+        assert hit and kind == (1 if vv < 0.75 else 0), (vv, kind)
+    beta = math.acos(math.sqrt(vh * vh - R * R) / vh)
+    assert zen[3] == pytest.approx((math.pi - beta) * 0.75, rel=1e-12)  # v = 0.375
+    assert zen[7] == pytest.approx(math.pi - beta + beta * 0.25, rel=1e-12)  # v = 0.875
+    for y in (2, 7):
+        toward, away = d(0, y), d(W - 1, y)
+        hor_t = toward - toward[1] * np.array([0, 1, 0])
+        hor_a = away - away[1] * np.array([0, 1, 0])
+        assert np.allclose(hor_t / np.linalg.norm(hor_t), fw, atol=1e-12)
+        assert np.allclose(hor_a / np.linalg.norm(hor_a), -fw, atol=1e-12)
+        u_side = (1 / math.sqrt(2)) * (W - 1) / W + 0.5 / W
+        s = d(0, y, u=u_side)
+        hor_s = s - s[1] * np.array([0, 1, 0])
+        assert np.allclose(hor_s / np.linalg.norm(hor_s), side, atol=1e-9)
+        mid = d(1, y)  # u = 0.5: cos(azimuth from the sun) = 0.5
+        hor_m = mid - mid[1] * np.array([0, 1, 0])
+        hor_m /= np.linalg.norm(hor_m)
+        assert hor_m @ fw == pytest.approx(0.5, abs=1e-12)
+        assert hor_m @ side == pytest.approx(math.sqrt(0.75), abs=1e-12)
