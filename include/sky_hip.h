@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define SKY_HIP_ABI_VERSION 3
+#define SKY_HIP_ABI_VERSION 4
 
 /* RenderError (integrator.rs:19-26) as status codes. */
 typedef enum sky_status {
@@ -150,6 +150,11 @@ typedef struct sky_render_stats {
     uint32_t workgroups;   /* launch shape actually used                */
     uint32_t block;
     uint32_t items;
+    /* sky_render_adjoint only: the objective sum dl_dxyz * XYZ over the
+     * replayed paths, summed in FP64 (the render's own XYZ image is FP32;
+     * this is the same estimator, for finite differences finer than FP32
+     * image rounding resolves) */
+    double adjoint_value;
 } sky_render_stats;
 
 /* Launch tuning of the wave-pool integrator (autotuner output); zero fields

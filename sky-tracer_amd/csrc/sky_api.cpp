@@ -141,7 +141,8 @@ sky_status run_render(sky_context *ctx, sky::HostScene &hs, hipStream_t stream,
         HIP_TRY(sky::launch_simple(s, stream, shape), "launch simple integrator", SKY_ERR_SYNC);
         if (s.xyz) HIP_TRY(sky::launch_film_to_xyz(s, stream), "launch xyz", SKY_ERR_SYNC);
     } else {
-        HIP_TRY(sky::launch_pool((const sky::DevJob *)ctx->jobs.p, 1, npx, plan, (uint32_t *)ctx->meta.p, 8, stream),
+        HIP_TRY(sky::launch_pool((const sky::DevJob *)ctx->jobs.p, 1, sky::finalize_blocks(ctx->host_jobs[0]), plan,
+                                 (uint32_t *)ctx->meta.p, 8, stream),
                 "launch wave-pool integrator", SKY_ERR_SYNC);
         shape.grid = plan.grid;
         shape.block = plan.block;
@@ -161,6 +162,7 @@ sky_status run_render(sky_context *ctx, sky::HostScene &hs, hipStream_t stream,
         stats->workgroups = shape.grid;
         stats->block = shape.block;
         stats->items = shape.items;
+        stats->adjoint_value = 0.0;
     }
     if (wd) return fail(SKY_ERR_WATCHDOG, "GPU path watchdog was reached");  // integrator.rs:301-304
     return SKY_OK;
@@ -411,7 +413,7 @@ sky_status adjoint_impl(sky_context *ctx, const sky_planet *planet, const sky_at
     HIP_TRY(ctx->meta.ensure(64), "alloc work counters", SKY_ERR_OOM);
     HIP_TRY(ctx->jobs.ensure(sizeof(sky::DevJob)), "alloc job table", SKY_ERR_OOM);
     HIP_TRY(ctx->adj_units.ensure(wl->count * sizeof(float4)), "alloc adjoint table", SKY_ERR_OOM);
-    HIP_TRY(ctx->adj_grad.ensure(SKY_NUM_PARAMS * sizeof(double)), "alloc gradient", SKY_ERR_OOM);
+    HIP_TRY(ctx->adj_grad.ensure((SKY_NUM_PARAMS + 1) * sizeof(double)), "alloc gradient", SKY_ERR_OOM);
     std::vector<float4> units(wl->count);
     for (uint32_t i = 0; i < wl->count; ++i) {
         double l = wl->center_nm[i], r = 550.0 / l, dl = (l - atm->ozone_lambda0_nm) / atm->ozone_sigma_nm;
@@ -427,7 +429,7 @@ sky_status adjoint_impl(sky_context *ctx, const sky_planet *planet, const sky_at
     HIP_TRY(hipMemcpyAsync(ctx->adj_units.p, units.data(), units.size() * sizeof(float4), hipMemcpyHostToDevice,
                            stream), "upload adjoint table", SKY_ERR_COPY);
     HIP_TRY(sky::launch_zero((uint32_t *)ctx->diag.p, 16, stream), "clear diagnostics", SKY_ERR_SYNC);
-    HIP_TRY(sky::launch_zero((uint32_t *)ctx->adj_grad.p, 2 * SKY_NUM_PARAMS, stream), "clear gradient",
+    HIP_TRY(sky::launch_zero((uint32_t *)ctx->adj_grad.p, 2 * (SKY_NUM_PARAMS + 1), stream), "clear gradient",
             SKY_ERR_SYNC);
     s.lam = (const sky::DevLambda *)ctx->lam.p;
     s.maj = (const float *)ctx->maj.p;
@@ -439,8 +441,7 @@ sky_status adjoint_impl(sky_context *ctx, const sky_planet *planet, const sky_at
     j.s = s;
     sky::plan_job(s, ctx->tuning, ctx->num_cus, sky::launch_min_items(ctx->tuning, ctx->num_cus), j);
     j.item_base = 0;
-    HIP_TRY(ctx->accum.ensure(sky::job_accum_bytes(j)), "alloc partial sums", SKY_ERR_OOM);
-    j.accum = (float2 *)ctx->accum.p;
+    j.accum = nullptr;  // the replay sums only the gradient
     j.done = (uint32_t *)ctx->meta.p + 4;
     j.t_done = (unsigned long long *)((uint32_t *)ctx->meta.p + 6);
     HIP_TRY(hipMemcpyAsync(ctx->jobs.p, &j, sizeof(j), hipMemcpyHostToDevice, stream), "upload job", SKY_ERR_COPY);
@@ -457,11 +458,14 @@ sky_status adjoint_impl(sky_context *ctx, const sky_planet *planet, const sky_at
             "launch adjoint", SKY_ERR_SYNC);
     HIP_TRY(hipEventRecord(ctx->ev1, stream), "event record", SKY_ERR_SYNC);
     uint32_t wd = 0;
-    HIP_TRY(hipMemcpyAsync(grad, ctx->adj_grad.p, SKY_NUM_PARAMS * sizeof(double), hipMemcpyDeviceToHost, stream),
+    double host_grad[SKY_NUM_PARAMS + 1];
+    HIP_TRY(hipMemcpyAsync(host_grad, ctx->adj_grad.p, (SKY_NUM_PARAMS + 1) * sizeof(double), hipMemcpyDeviceToHost,
+                           stream),
             "copy gradient", SKY_ERR_COPY);
     HIP_TRY(hipMemcpyAsync(&wd, ctx->diag.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream), "read diagnostics",
             SKY_ERR_COPY);
     HIP_TRY(hipStreamSynchronize(stream), "synchronize", SKY_ERR_SYNC);
+    std::memcpy(grad, host_grad, SKY_NUM_PARAMS * sizeof(double));
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
     if (stats) {
@@ -471,6 +475,7 @@ sky_status adjoint_impl(sky_context *ctx, const sky_planet *planet, const sky_at
         stats->workgroups = plan.grid;
         stats->block = plan.block;
         stats->items = plan.n_items;
+        stats->adjoint_value = host_grad[SKY_NUM_PARAMS];
     }
     if (wd) return fail(SKY_ERR_WATCHDOG, "GPU path watchdog was reached");
     return SKY_OK;
@@ -548,7 +553,7 @@ sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint3
 
     // Table offsets.
     size_t n_lam = 0, n_maj = 0, n_shl = 0, n_xyz = 0, n_film = 0, n_acc = 0, n_tab = 0;
-    uint32_t max_npx = 0, total_items = 0, max_maj = 0, max_layers = 0, flags_all = 0;
+    uint32_t max_npx = 0, total_items = 0, max_maj = 0, max_layers = 0, flags_all = 0, fin_blocks = 0;
     std::vector<size_t> o_lam(n), o_maj(n), o_shl(n), o_xyz(n), o_film(n), o_acc(n), o_tab(n);
     ctx->host_jobs.assign(n, sky::DevJob{});
     const uint32_t min_items = 0;  // the batch balances across renders; no sample split needed
@@ -577,6 +582,7 @@ sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint3
         o_acc[i] = n_acc;
         n_acc += sky::job_accum_bytes(j) / sizeof(float2);
         max_npx = npx > max_npx ? npx : max_npx;
+        fin_blocks = std::max(fin_blocks, sky::finalize_blocks(j));
         max_maj = h.maj.size() > max_maj ? (uint32_t)h.maj.size() : max_maj;
         max_layers = h.dev.L > max_layers ? h.dev.L : max_layers;
         flags_all = k == 0 || h.dev.flags == flags_all ? h.dev.flags : sky::kFlagMixed;
@@ -644,6 +650,7 @@ sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint3
     mix(n);
     mix(total_items);
     mix(max_npx);
+    mix(fin_blocks);
     mix((uint64_t)(uintptr_t)ctx->b_jobs.p);
     mix((uint64_t)(uintptr_t)ctx->b_meta.p);
     mix(plan.grid);
@@ -661,7 +668,7 @@ sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint3
             ctx->graph_exec = nullptr;
             ctx->graph = nullptr;
             HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture", SKY_ERR_SYNC);
-            hipError_t le = sky::launch_pool((const sky::DevJob *)ctx->b_jobs.p, n, max_npx, plan, meta,
+            hipError_t le = sky::launch_pool((const sky::DevJob *)ctx->b_jobs.p, n, fin_blocks, plan, meta,
                                              meta_words, stream);
             hipError_t ce = hipStreamEndCapture(stream, &ctx->graph);
             if (le != hipSuccess) return hip_fail(le, "capture batch launch", SKY_ERR_SYNC);
@@ -674,7 +681,7 @@ sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint3
         }
         HIP_TRY(hipGraphLaunch(ctx->graph_exec, stream), "launch graph", SKY_ERR_SYNC);
     } else {
-        HIP_TRY(sky::launch_pool((const sky::DevJob *)ctx->b_jobs.p, n, max_npx, plan, meta, meta_words, stream),
+        HIP_TRY(sky::launch_pool((const sky::DevJob *)ctx->b_jobs.p, n, fin_blocks, plan, meta, meta_words, stream),
                 "launch batch", SKY_ERR_SYNC);
     }
     HIP_TRY(hipEventRecord(ctx->ev1, stream), "event record", SKY_ERR_SYNC);

@@ -79,7 +79,7 @@ hipError_t launch_film_to_xyz(const DevScene &s, hipStream_t stream);
 struct DevJob {
     DevScene s;
     uint32_t pixels_per_item, groups, chunks, samples_per_chunk, n_items, item_base;
-    float2 *accum;              // [chunk][pixel][band] (sum, sum of squares)
+    float2 *accum;              // [chunk][pixel][band] (sum, sum of squares); only when chunks > 1
     uint32_t *done;             // items finished
     unsigned long long *t_done; // device clock when the last item finished
 };
@@ -101,7 +101,7 @@ struct AdjArgs {
     const float4 *lam_units;  // per band: (beta_R, ozone, aerosol) per unit parameter, ln(l / 550)
     float inv_hr2, inv_ha2;   // 1 / H^2
     float inv_spp;
-    double *grad;             // [10], accumulated atomically
+    double *grad;             // [10] gradient, [10] the objective (sum dLoss/dXYZ * XYZ), atomically
 };
 
 // Launch-wide settings of the wave-pool integrator (sky_wavepool.hip).
@@ -116,6 +116,7 @@ struct WavePlan {
 
 void plan_job(const DevScene &s, const sky_tuning &t, int num_cus, uint32_t min_items, DevJob &j);
 size_t job_accum_bytes(const DevJob &j);
+uint32_t finalize_blocks(const DevJob &j);
 uint32_t launch_min_items(const sky_tuning &t, int num_cus);
 // max_maj: the largest nlam x majorant_layers over the launch's jobs;
 // max_layers: the largest majorant_layers; flags_all: the estimator flags if
@@ -129,7 +130,9 @@ constexpr uint32_t kFlagMixed = 0x80000000u;
 constexpr uint32_t kFastMaxShells = 8;
 WavePlan plan_launch(const sky_tuning &t, int num_cus, uint32_t total_items, uint32_t max_maj,
                      uint32_t max_layers, uint32_t flags_all);
-hipError_t launch_pool(const DevJob *d_jobs, uint32_t n_jobs, uint32_t max_npx, const WavePlan &p,
+// fin_blocks: the largest finalize_blocks over the launch's jobs (0: no
+// job was split into sample chunks, no finalize launch).
+hipError_t launch_pool(const DevJob *d_jobs, uint32_t n_jobs, uint32_t fin_blocks, const WavePlan &p,
                        uint32_t *meta, uint32_t meta_words, hipStream_t stream);
 hipError_t launch_zero(uint32_t *p, uint32_t n, hipStream_t stream);
 hipError_t launch_adjoint(const DevJob *d_jobs, const WavePlan &p, uint32_t *meta, uint32_t meta_words,

@@ -99,6 +99,7 @@ constexpr int kNP = 10;
 
 struct AdjLane {
     float S[kNP], SS[kNP], G[kNP];
+    float J;         // this path's share of the objective sum dLoss/dXYZ * XYZ
     float w;         // dLoss / dL of this path's (pixel, band), divided by spp
     float vsr, vsas; // sigma_R and sigma_As at the scattering vertex
     float vh;        // altitude of the scattering vertex
@@ -188,8 +189,10 @@ __device__ __forceinline__ void add_dlog_phase(const DevScene &s, const AdjArgs 
 // removes a global-load latency from every shell crossing.
 constexpr uint32_t kMajLdsMax = 2048;
 // Per-wave LDS layout (floats): sums [256], sums of squares [256], adjoint
-// gradient [16], majorant grid [maj_lds], shell table [shell_lds doubles].
-constexpr uint32_t kLdsFixed = 2 * kSlotsPerWave + 16;
+// gradient [16 doubles], majorant grid [maj_lds], shell table [shell_lds
+// doubles].  The paths' gradient contributions cancel heavily across paths
+// (scores of both signs), so the item's sum is kept in FP64.
+constexpr uint32_t kLdsFixed = 2 * kSlotsPerWave + 32;
 
 typedef __attribute__((address_space(3))) const float *LdsFloatPtr;
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -476,14 +479,15 @@ enum Next : int { NX_NEE = 1, NX_CONT = 2, NX_MAIN = 3, NX_FINISH = 4 };
 // Path end: the path's radiance and its square into the wave's LDS sums.
 template <bool ADJ>
 __device__ __forceinline__ void finish_path(LaneCtx &c, AdjLane &a, float *acc_sum, float *acc_sq,
-                                            float *acc_grad)
+                                            double *acc_grad)
 {
     float Lf = fmaxf(c.L, 0.0f);
     atomicAdd(&acc_sum[c.slot], Lf);
     atomicAdd(&acc_sq[c.slot], Lf * Lf);
     if constexpr (ADJ) {
         for (int k = 0; k < kNP; ++k)
-            if (a.G[k] != 0.0f && a.G[k] == a.G[k]) atomicAdd(&acc_grad[k], a.G[k]);
+            if (a.G[k] != 0.0f && a.G[k] == a.G[k]) atomicAdd(&acc_grad[k], (double)a.G[k]);
+        if (a.J != 0.0f && a.J == a.J) atomicAdd(&acc_grad[kNP], (double)a.J);
     }
     c.st = ST_NEED;
 }
@@ -498,7 +502,7 @@ __device__ __forceinline__ void finish_path(LaneCtx &c, AdjLane &a, float *acc_s
 // NEED (path finished and accumulated).
 template <bool ADJ>
 __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m, LaneCtx &c, AdjLane &a,
-                                             const AdjArgs &A, float *acc_sum, float *acc_sq, float *acc_grad,
+                                             const AdjArgs &A, float *acc_sum, float *acc_sq, double *acc_grad,
                                              bool &wd)
 {
     const uint32_t n_light = s.dls;
@@ -516,6 +520,7 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
                 c.L = c.L + C;
                 if constexpr (ADJ) {
                     for (int k = 0; k < kNP; ++k) a.G[k] += a.w * C * a.S[k];
+                    a.J += a.w * C;
                 }
             }
         } else if (c.kind == 0) {
@@ -540,8 +545,10 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
         c.nee_sum += c.nee_factor * c.aux;
         if constexpr (ADJ) {
             float C = c.tp * c.nee_factor * c.aux / (float)n_light;
-            if (C != 0.0f)
+            if (C != 0.0f) {
                 for (int k = 0; k < kNP; ++k) a.G[k] += a.w * C * a.SS[k];
+                a.J += a.w * C;
+            }
         }
         c.nee_i++;
         nx = c.nee_i < n_light ? NX_NEE : NX_CONT;
@@ -660,6 +667,7 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
                     c.L = 0.0f;
                     if constexpr (ADJ) {
                         for (int k = 0; k < kNP; ++k) a.G[k] = 0.0f;
+                        a.J = 0.0f;
                     }
                 } else {
                     c.depth++;
@@ -703,7 +711,7 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
 // (within 1 ulp of the quotient).
 template <bool ADJ, bool IS, bool RATIO>
 __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSrc &m, LaneCtx &c, AdjLane &a,
-                                                  const AdjArgs &A, float *acc_sum, float *acc_sq, float *acc_grad,
+                                                  const AdjArgs &A, float *acc_sum, float *acc_sq, double *acc_grad,
                                                   bool &wd)
 {
     const uint32_t n_light = s.dls;
@@ -731,6 +739,7 @@ __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSr
         const bool gh = shd && Csh != 0.0f;
         for (int k = 0; k < kNP; ++k)
             a.G[k] += (sun_seen ? a.w * Csun * a.S[k] : 0.0f) + (gh ? a.w * Csh * a.SS[k] : 0.0f);
+        a.J += (sun_seen ? a.w * Csun : 0.0f) + (gh ? a.w * Csh : 0.0f);
     }
     c.L = sun_seen ? c.L + c.tp * lsun : c.L;
     wd = wd || (bnd && c.kind == 3);
@@ -823,6 +832,7 @@ __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSr
             a.S[k] = is_cont ? a.S[k] + t : a.S[k];
             a.G[k] = is_cont && live && bad ? 0.0f : a.G[k];
         }
+        a.J = is_cont && live && bad ? 0.0f : a.J;
     }
     if (is_cont) {
         c.L = live && bad ? 0.0f : Lc;
@@ -954,6 +964,40 @@ __device__ __forceinline__ void start_path(const DevScene &s, LaneCtx &c, AdjLan
             a.S[k] = 0.0f;
             a.G[k] = 0.0f;
         }
+        a.J = 0.0f;
+    }
+}
+
+// End of a work item that holds every sample of its np pixels: the film
+// (1/spp, clamped at 0, integrator.rs:307-310), its second moment and
+// XYZ = sum_band film * cmf * weight, straight from the wave's LDS sums.  The
+// film rows of the item's pixels are contiguous ([pixel][band]), so the
+// stores coalesce; one lane per pixel sums its bands in band order.
+__device__ __forceinline__ void item_outputs(const DevScene &s, uint32_t p0, uint32_t np, uint32_t nslots,
+                                             const float *sum, const float *sq)
+{
+    const float inv = 1.0f / (float)s.spp;
+    const uint32_t lane = lane_id();
+    const size_t base = (size_t)p0 * s.nlam;
+    if (s.film)
+        for (uint32_t i = lane; i < nslots; i += 64) s.film[base + i] = fmaxf(sum[i] * inv, 0.0f);
+    if (s.film_m2)
+        for (uint32_t i = lane; i < nslots; i += 64) s.film_m2[base + i] = sq[i] * inv;
+    if (s.xyz) {
+        for (uint32_t pl = lane; pl < np; pl += 64) {
+            float X = 0.0f, Y = 0.0f, Z = 0.0f;
+            for (uint32_t b = 0; b < s.nlam; ++b) {
+                const float f = fmaxf(sum[pl * s.nlam + b] * inv, 0.0f);
+                const DevLambda l = s.lam[b];
+                X += f * l.wx;
+                Y += f * l.wy;
+                Z += f * l.wz;
+            }
+            float *o = s.xyz + 3 * (size_t)(p0 + pl);
+            o[0] = X;
+            o[1] = Y;
+            o[2] = Z;
+        }
     }
 }
 
@@ -986,7 +1030,7 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
     float *wave_lds = dyn_lds + wid * (kLdsFixed + a.maj_lds + 2 * a.shell_lds + a.lam_lds);
     float *acc_sum = wave_lds;
     float *acc_sq = wave_lds + kSlotsPerWave;
-    float *acc_grad = wave_lds + 2 * kSlotsPerWave;
+    double *acc_grad = (double *)(wave_lds + 2 * kSlotsPerWave);
     float *lds_maj = wave_lds + kLdsFixed;
     double *lds_shell = (double *)(lds_maj + a.maj_lds);
     // FAST: [band] 4-float coefficients, then [band] sun radiance (kMaxLambda bands each)
@@ -1050,7 +1094,7 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
             }
         }
         if constexpr (ADJ) {
-            if (lane < kNP) acc_grad[lane] = 0.0f;
+            if (lane <= kNP) acc_grad[lane] = 0.0;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -1095,10 +1139,14 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        float2 *dst = J.accum + ((size_t)ch * npx + it.p0) * s.nlam;
-        for (uint32_t i = lane; i < it.nslots; i += 64) dst[i] = make_float2(acc_sum[i], acc_sq[i]);
         if constexpr (ADJ) {
-            if (lane < kNP) atomicAdd(&A.grad[lane], (double)acc_grad[lane]);
+            if (lane <= kNP) atomicAdd(&A.grad[lane], acc_grad[lane]);
+        } else if (J.chunks == 1) {
+            // the item holds every sample of its pixels: final values now
+            item_outputs(s, it.p0, np, it.nslots, acc_sum, acc_sq);
+        } else {
+            float2 *dst = J.accum + ((size_t)ch * npx + it.p0) * s.nlam;
+            for (uint32_t i = lane; i < it.nslots; i += 64) dst[i] = make_float2(acc_sum[i], acc_sq[i]);
         }
         __builtin_amdgcn_wave_barrier();
         if (lane == 0) {
@@ -1157,38 +1205,63 @@ SKY_ADJOINT(k_wavepool_adjoint_fastref, 2)
 SKY_ADJOINT(k_wavepool_adjoint_fastrt, 3)
 #undef SKY_ADJOINT
 
-// Per job (blockIdx.y): reduces the chunk partials in chunk order,
-// normalises by 1/spp and clamps (integrator.rs:307-310), and forms
-// XYZ = sum_band film * cmf * weight.
+// Per job (blockIdx.y) whose samples were split into several chunks: reduces
+// the chunk partials in chunk order, normalises by 1/spp and clamps
+// (integrator.rs:307-310), and forms XYZ = sum_band film * cmf * weight.  A
+// block takes P = 256 / nlam whole pixels: thread t owns entry t of their
+// [pixel][band] rows, so every chunk's partials are read in one coalesced
+// sweep; the block's film values then sit in LDS for the per-pixel XYZ sums
+// (band order, as item_outputs).  Jobs done in one chunk wrote their outputs
+// at the end of their items.
 __global__ __launch_bounds__(256) void k_finalize(const DevJob *jobs)
 {
+    __shared__ float sf[256];
     const DevJob &J = job_at(jobs, blockIdx.y);
+    if (J.chunks <= 1) return;
     const DevScene &s = J.s;
-    uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t npx = s.width * s.height;
-    if (pixel >= npx) return;
-    float inv = 1.0f / (float)s.spp;
-    float X = 0.0f, Y = 0.0f, Z = 0.0f;
-    for (uint32_t b = 0; b < s.nlam; ++b) {
+    const uint32_t npx = s.width * s.height, nl = s.nlam;
+    const uint32_t P = 256u / nl;
+    const uint32_t px0 = blockIdx.x * P;
+    if (px0 >= npx) return;
+    const uint32_t npix = min(P, npx - px0);
+    const uint32_t t = threadIdx.x;
+    const float inv = 1.0f / (float)s.spp;
+    if (t < npix * nl) {
+        const size_t e = (size_t)px0 * nl + t;
         float sum = 0.0f, sq = 0.0f;
         for (uint32_t ch = 0; ch < J.chunks; ++ch) {
-            float2 v = J.accum[((size_t)ch * npx + pixel) * s.nlam + b];
+            const float2 v = J.accum[(size_t)ch * npx * nl + e];
             sum += v.x;
             sq += v.y;
         }
-        float f = fmaxf(sum * inv, 0.0f);
-        if (s.film) s.film[(size_t)pixel * s.nlam + b] = f;
-        if (s.film_m2) s.film_m2[(size_t)pixel * s.nlam + b] = sq * inv;
-        DevLambda l = s.lam[b];
-        X += f * l.wx;
-        Y += f * l.wy;
-        Z += f * l.wz;
+        const float f = fmaxf(sum * inv, 0.0f);
+        sf[t] = f;
+        if (s.film) s.film[e] = f;
+        if (s.film_m2) s.film_m2[e] = sq * inv;
     }
-    if (s.xyz) {
-        s.xyz[3 * pixel + 0] = X;
-        s.xyz[3 * pixel + 1] = Y;
-        s.xyz[3 * pixel + 2] = Z;
+    __syncthreads();
+    if (s.xyz && t < npix) {
+        float X = 0.0f, Y = 0.0f, Z = 0.0f;
+        for (uint32_t b = 0; b < nl; ++b) {
+            const float f = sf[t * nl + b];
+            const DevLambda l = s.lam[b];
+            X += f * l.wx;
+            Y += f * l.wy;
+            Z += f * l.wz;
+        }
+        float *o = s.xyz + 3 * (size_t)(px0 + t);
+        o[0] = X;
+        o[1] = Y;
+        o[2] = Z;
     }
+}
+
+// Blocks of k_finalize for a job (0 when the job needs none).
+uint32_t finalize_blocks(const DevJob &j)
+{
+    if (j.chunks <= 1) return 0;
+    const uint32_t P = 256u / j.s.nlam;
+    return (j.s.width * j.s.height + P - 1) / P;
 }
 
 // Fills n words with zero: used instead of a memset node so that the launch
@@ -1230,8 +1303,11 @@ void plan_job(const DevScene &s, const sky_tuning &t, int num_cus, uint32_t min_
     j.n_items = j.groups * j.chunks;
 }
 
+// Partial-sum buffer of a job: only jobs split into several sample chunks
+// need one (a single-chunk item writes its pixels' outputs itself).
 size_t job_accum_bytes(const DevJob &j)
 {
+    if (j.chunks <= 1) return 0;
     return (size_t)j.chunks * j.s.width * j.s.height * j.s.nlam * sizeof(float2);
 }
 
@@ -1270,7 +1346,7 @@ uint32_t launch_min_items(const sky_tuning &t, int num_cus)
 
 // meta: [0] item counter, [1..2] launch timestamp, [4 + 2 j ..] per-job done
 // counter / pad, then per-job finish timestamps; zeroed by the first kernel.
-hipError_t launch_pool(const DevJob *d_jobs, uint32_t n_jobs, uint32_t max_npx, const WavePlan &p,
+hipError_t launch_pool(const DevJob *d_jobs, uint32_t n_jobs, uint32_t fin_blocks, const WavePlan &p,
                        uint32_t *meta, uint32_t meta_words, hipStream_t stream)
 {
     hipLaunchKernelGGL(k_zero_u32, dim3((meta_words + 255) / 256), dim3(256), 0, stream, meta, meta_words);
@@ -1297,7 +1373,8 @@ hipError_t launch_pool(const DevJob *d_jobs, uint32_t n_jobs, uint32_t max_npx, 
     hipLaunchKernelGGL(k, dim3(p.grid), dim3(p.block), p.lds_bytes, stream, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_finalize, dim3((max_npx + 255) / 256, n_jobs), dim3(256), 0, stream, d_jobs);
+    if (fin_blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_finalize, dim3(fin_blocks, n_jobs), dim3(256), 0, stream, d_jobs);
     return hipGetLastError();
 }
 

@@ -110,6 +110,7 @@ class RenderStats(C.Structure):
         ("workgroups", C.c_uint32),
         ("block", C.c_uint32),
         ("items", C.c_uint32),
+        ("adjoint_value", C.c_double),
     ]
 
 

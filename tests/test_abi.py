@@ -38,6 +38,7 @@ def test_struct_sizes_match_header_layout():
     assert C.sizeof(L.Atmosphere) == 15 * 8 + 8
     assert C.sizeof(L.MediumTablesC) == 8 + 6 * 8
     assert C.sizeof(L.Tuning) == 5 * 4
+    assert C.sizeof(L.RenderStats) == 2 * 8 + 8 + 4 * 4 + 8
 
 
 def test_defaults_match_reference():
