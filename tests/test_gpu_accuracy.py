@@ -4,9 +4,10 @@ tests/golden/c2_fp64_reference.npz holds the double-precision oracle's XYZ at
 the 64x32 pixels x = 16 i + 8, y = 16 j + 8 of the 1024x512 panorama (sun 5 deg,
 depth <= 32, 65536 samples per pixel, written by make_c2_reference.py) with
 their standard errors.  The GPU renders the whole image at 1024 samples per
-pixel with the benchmark's estimator (carried optical depth, lobe sampling, 8
-majorant shells; the oracle used 16 shells and ratio-tracked shadow rays --
-the same expectation).  At 1024 spp the Monte Carlo noise of a single pixel
+pixel with the benchmark's estimator (carried optical depth, lobe sampling)
+under the benchmark's tuned launch (3 shells growing 4x, k_wavepool_fast_o6)
+and with 8 uniform shells; the oracle used 16 shells and ratio-tracked shadow
+rays -- the same expectation.  At 1024 spp the Monte Carlo noise of a single pixel
 is ~1.3% of its value, so the test separates noise from error: per-value z
 scores with both variances, and the bias left after the expected noise is
 taken out of the mean squared difference must stay below 0.5% of the image's
@@ -25,14 +26,27 @@ pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c2_fp64_reference.npz")
 
 
-def test_full_render_matches_fp64_oracle(gpu_context):
+@pytest.mark.parametrize("launch", ["tuned", "8 shells"])
+def test_full_render_matches_fp64_oracle(gpu_context, launch):
+    """"tuned": exactly what bench.py times -- the tuning cache's shells (3,
+    growing 4x), register budget (6 waves per SIMD: k_wavepool_fast_o6) and
+    launch shape; "8 shells": 8 uniform shells, the default launch."""
+    from pysky import autotune
+
     ref = np.load(GOLDEN)
     # seed 2: the FP64 render used seed 1, whose first samples would share random numbers
     scene, cfg, _ = case(width=1024, height=512, n_lambda=64, spp=1024, sun=5.0, max_depth=32,
                          mode="carried_is", layers=8, seed=2)
     assert np.allclose(scene.wavelengths_nm, ref["wavelengths_nm"])
-    print(f"[gpu] render 1024x512 nl=64 spp=1024 carried_is L8", flush=True)
-    f = pysky.render(scene, cfg, want_film=True, want_m2=True, want_xyz=True, context=gpu_context)
+    ctx = gpu_context
+    if launch == "tuned":
+        ctx = pysky.Context(0)
+        params = autotune.tuned(scene, cfg)
+        assert params["majorant_layers"] == 3 and params["shell_growth"] == 4.0 and params["occupancy"] == 6
+        cfg = autotune.apply(ctx, cfg, params)
+    print(f"[gpu] render 1024x512 nl=64 spp=1024 carried_is, {launch}: L{cfg.majorant_layers} "
+          f"growth {cfg.shell_growth}", flush=True)
+    f = pysky.render(scene, cfg, want_film=True, want_m2=True, want_xyz=True, context=ctx)
     nb = 64
     pix = ref["pixels"]
     film = f.values().reshape(-1, nb)[pix].astype(np.float64)
