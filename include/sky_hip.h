@@ -212,6 +212,27 @@ sky_status sky_render_device(sky_context *ctx, const sky_planet *planet, const s
                              const sky_wavelengths *wl, const sky_render_config *cfg, float *d_film,
                              float *d_film_m2, float *d_xyz, void *stream, sky_render_stats *stats);
 
+/* ---- allocation-free render ----
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  Same inputs, outputs and results as sky_render_device
+ * (outputs are the caller's device buffers), but the working buffers -- the
+ * packed scene tables, work counters, job record and, when the render's
+ * samples are split into chunks, the chunk partial sums -- live in a static
+ * device array of SKY_NOALLOC_SCRATCH_BYTES that the library holds on every
+ * GPU it runs on until the process ends; the call makes no device
+ * allocation.  If the usual sample-chunk partials do not fit, the render
+ * runs one chunk per pixel group; a render whose tables alone, or whose
+ * samples per pixel (beyond 2^24 / (pixels per item x count)) need more,
+ * fails with SKY_ERR_INVALID_CONFIG.  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (other threads, other
+ * contexts, other streams) race and must be serialised by the caller.
+ * Synchronous on return, ordered on `stream` (NULL: the context's). */
+#define SKY_NOALLOC_SCRATCH_BYTES (4u << 20)
+sky_status sky_render_device_noalloc(sky_context *ctx, const sky_planet *planet, const sky_atmosphere *atm,
+                                     const sky_wavelengths *wl, const sky_render_config *cfg, float *d_film,
+                                     float *d_film_m2, float *d_xyz, void *stream, sky_render_stats *stats);
+
 /* ---- adjoint: gradient of the rendered XYZ image ----
  * Returns grad[k] = d/dtheta_k sum_{pixel,c} dl_dxyz[pixel*3+c] * XYZ[pixel*3+c]
  * for the ten parameters, in this order: rayleigh_sca_550_m_inv,

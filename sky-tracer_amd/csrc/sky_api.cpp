@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -20,6 +21,33 @@
 namespace {
 
 thread_local std::string g_last_error;
+
+// Working memory of sky_render_device_noalloc: one copy per GPU, held until
+// the process ends.  Host code never passes `scratch` itself to a kernel or
+// a copy (in host code the name is only a stand-in for the device symbol):
+// each context looks up its device's address with hipGetSymbolAddress when
+// it is created and uses that pointer.
+static __device__ __align__(256) uint8_t scratch[SKY_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+uint8_t *g_scratch_ptr[kMaxDevices] = {};
+
+// The device's scratch address, looked up once per device with that device
+// current (the caller has made it current).
+hipError_t scratch_address(int device, uint8_t **out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = (uint8_t *)p;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
 
 sky_status fail(sky_status st, const std::string &msg)
 {
@@ -52,6 +80,7 @@ struct DevBuf {
 
 struct sky_context {
     int device = 0;
+    uint8_t *scratch = nullptr;  // this device's sky_render_device_noalloc scratch (device address)
     int num_cus = 256;
     double clock_khz = 100000.0;  // wall_clock64 rate
     hipStream_t stream = nullptr;
@@ -82,67 +111,122 @@ sky_status hip_fail(hipError_t e, const char *what, sky_status st = SKY_ERR_SYNC
         if (e_ != hipSuccess) return hip_fail(e_, what, st);         \
     } while (0)
 
-// Runs one render into device buffers; outputs already set in hs.dev.
-sky_status run_render(sky_context *ctx, sky::HostScene &hs, hipStream_t stream,
-                      sky_render_stats *stats)
+// Device buffers of one render.
+struct RenderBufs {
+    void *lam = nullptr, *maj = nullptr, *shells = nullptr, *diag = nullptr, *meta = nullptr, *job = nullptr,
+         *tab = nullptr, *accum = nullptr;
+};
+
+// Carves the buffers of a render from the allocation-free entry point's
+// scratch array (every offset 256-byte aligned); false if they do not fit.
+bool carve_scratch(uint8_t *base, const sky::HostScene &hs, size_t accum_bytes, size_t tab_bytes, RenderBufs &b)
+{
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        void *p = base + off;
+        off += (bytes + 255) & ~size_t(255);
+        return p;
+    };
+    b.meta = take(64 * sizeof(uint32_t));
+    b.diag = take(16 * sizeof(uint32_t));
+    b.job = take(sizeof(sky::DevJob));
+    b.lam = take(hs.lam.size() * sizeof(sky::DevLambda));
+    b.maj = take(hs.maj.size() * sizeof(float));
+    b.shells = take(hs.shell_d.size() * sizeof(double));
+    b.tab = tab_bytes ? take(tab_bytes) : nullptr;
+    b.accum = accum_bytes ? take(accum_bytes) : nullptr;
+    return off <= SKY_NOALLOC_SCRATCH_BYTES;
+}
+
+// Runs one render into device buffers; outputs already set in hs.dev.  The
+// working buffers are the context's grow-only device buffers, or with
+// `scratch` (the allocation-free entry point) carved from that device
+// array: then nothing is allocated on the device, the work split is the
+// usual one when its partial sums fit, else one sample chunk per pixel group.
+sky_status run_render(sky_context *ctx, sky::HostScene &hs, hipStream_t stream, sky_render_stats *stats,
+                      uint8_t *scratch = nullptr)
 {
     sky::DevScene &s = hs.dev;
-    HIP_TRY(ctx->lam.ensure(hs.lam.size() * sizeof(sky::DevLambda)), "alloc lambda table", SKY_ERR_OOM);
-    HIP_TRY(ctx->maj.ensure(hs.maj.size() * sizeof(float)), "alloc majorants", SKY_ERR_OOM);
-    HIP_TRY(ctx->shells.ensure(hs.shell_d.size() * sizeof(double)), "alloc shell table", SKY_ERR_OOM);
-    HIP_TRY(ctx->diag.ensure(64), "alloc diagnostics", SKY_ERR_OOM);
-    HIP_TRY(ctx->meta.ensure(64), "alloc work counters", SKY_ERR_OOM);
-    HIP_TRY(ctx->jobs.ensure(sizeof(sky::DevJob)), "alloc job table", SKY_ERR_OOM);
-    HIP_TRY(hipMemcpyAsync(ctx->lam.p, hs.lam.data(), hs.lam.size() * sizeof(sky::DevLambda),
-                           hipMemcpyHostToDevice, stream),
-            "upload lambda table", SKY_ERR_COPY);
-    HIP_TRY(hipMemcpyAsync(ctx->maj.p, hs.maj.data(), hs.maj.size() * sizeof(float),
-                           hipMemcpyHostToDevice, stream),
-            "upload majorants", SKY_ERR_COPY);
-    HIP_TRY(hipMemcpyAsync(ctx->shells.p, hs.shell_d.data(), hs.shell_d.size() * sizeof(double),
-                           hipMemcpyHostToDevice, stream),
-            "upload shell table", SKY_ERR_COPY);
-    HIP_TRY(sky::launch_zero((uint32_t *)ctx->diag.p, 16, stream), "clear diagnostics", SKY_ERR_SYNC);
-    if (!hs.tab.empty()) {
-        hs.tab.pack(ctx->host_tab);
-        HIP_TRY(ctx->tab.ensure(ctx->host_tab.size()), "alloc medium table", SKY_ERR_OOM);
-        HIP_TRY(hipMemcpyAsync(ctx->tab.p, ctx->host_tab.data(), ctx->host_tab.size(), hipMemcpyHostToDevice, stream),
-                "upload medium table", SKY_ERR_COPY);
-    }
-    hs.tab.attach(s, ctx->tab.p);
-    s.lam = (const sky::DevLambda *)ctx->lam.p;
-    s.maj = (const float *)ctx->maj.p;
-    s.shell_d = (const double *)ctx->shells.p;
-    s.diag = (uint32_t *)ctx->diag.p;
     const bool simple = (s.flags & SKY_FLAG_SIMPLE_KERNEL) != 0;
-    sky::LaunchShape shape;
-    sky::WavePlan plan;
     const uint32_t npx = s.width * s.height;
+    sky::DevJob j{};
     if (!simple) {
-        ctx->host_jobs.assign(1, sky::DevJob{});
-        sky::DevJob &j = ctx->host_jobs[0];
         j.s = s;
         sky::plan_job(s, ctx->tuning, ctx->num_cus, sky::launch_min_items(ctx->tuning, ctx->num_cus), j);
+    }
+    if (!hs.tab.empty()) hs.tab.pack(ctx->host_tab);
+    const size_t tab_bytes = hs.tab.empty() ? 0 : ctx->host_tab.size();
+    RenderBufs b;
+    if (scratch) {
+        if (simple) return fail(SKY_ERR_INVALID_CONFIG, "the allocation-free entry point runs the wave-pool integrator");
+        if (!carve_scratch(scratch, hs, sky::job_accum_bytes(j), tab_bytes, b)) {
+            sky::plan_job(s, ctx->tuning, ctx->num_cus, 0, j);  // one chunk per pixel group when spp allows
+            if (!carve_scratch(scratch, hs, sky::job_accum_bytes(j), tab_bytes, b))
+                return fail(SKY_ERR_INVALID_CONFIG,
+                            "render needs more than the allocation-free entry point's " +
+                                std::to_string(SKY_NOALLOC_SCRATCH_BYTES >> 20) +
+                                " MiB of scratch (tables plus sample-chunk partials; fewer samples per pixel, "
+                                "wavelengths or medium breakpoints fit)");
+        }
+    } else {
+        HIP_TRY(ctx->lam.ensure(hs.lam.size() * sizeof(sky::DevLambda)), "alloc lambda table", SKY_ERR_OOM);
+        HIP_TRY(ctx->maj.ensure(hs.maj.size() * sizeof(float)), "alloc majorants", SKY_ERR_OOM);
+        HIP_TRY(ctx->shells.ensure(hs.shell_d.size() * sizeof(double)), "alloc shell table", SKY_ERR_OOM);
+        HIP_TRY(ctx->diag.ensure(64), "alloc diagnostics", SKY_ERR_OOM);
+        HIP_TRY(ctx->meta.ensure(64), "alloc work counters", SKY_ERR_OOM);
+        HIP_TRY(ctx->jobs.ensure(sizeof(sky::DevJob)), "alloc job table", SKY_ERR_OOM);
+        if (tab_bytes) HIP_TRY(ctx->tab.ensure(tab_bytes), "alloc medium table", SKY_ERR_OOM);
+        if (!simple) HIP_TRY(ctx->accum.ensure(sky::job_accum_bytes(j)), "alloc partial sums", SKY_ERR_OOM);
+        if (simple && !s.film) {
+            HIP_TRY(ctx->film.ensure((size_t)npx * s.nlam * sizeof(float)), "alloc film", SKY_ERR_OOM);
+            s.film = (float *)ctx->film.p;
+        }
+        b.lam = ctx->lam.p;
+        b.maj = ctx->maj.p;
+        b.shells = ctx->shells.p;
+        b.diag = ctx->diag.p;
+        b.meta = ctx->meta.p;
+        b.job = ctx->jobs.p;
+        b.tab = ctx->tab.p;
+        b.accum = ctx->accum.p;
+    }
+    HIP_TRY(hipMemcpyAsync(b.lam, hs.lam.data(), hs.lam.size() * sizeof(sky::DevLambda), hipMemcpyHostToDevice,
+                           stream),
+            "upload lambda table", SKY_ERR_COPY);
+    HIP_TRY(hipMemcpyAsync(b.maj, hs.maj.data(), hs.maj.size() * sizeof(float), hipMemcpyHostToDevice, stream),
+            "upload majorants", SKY_ERR_COPY);
+    HIP_TRY(hipMemcpyAsync(b.shells, hs.shell_d.data(), hs.shell_d.size() * sizeof(double), hipMemcpyHostToDevice,
+                           stream),
+            "upload shell table", SKY_ERR_COPY);
+    HIP_TRY(sky::launch_zero((uint32_t *)b.diag, 16, stream), "clear diagnostics", SKY_ERR_SYNC);
+    if (tab_bytes)
+        HIP_TRY(hipMemcpyAsync(b.tab, ctx->host_tab.data(), tab_bytes, hipMemcpyHostToDevice, stream),
+                "upload medium table", SKY_ERR_COPY);
+    hs.tab.attach(s, b.tab);
+    s.lam = (const sky::DevLambda *)b.lam;
+    s.maj = (const float *)b.maj;
+    s.shell_d = (const double *)b.shells;
+    s.diag = (uint32_t *)b.diag;
+    sky::LaunchShape shape;
+    sky::WavePlan plan;
+    if (!simple) {
+        j.s = s;
         j.item_base = 0;
-        HIP_TRY(ctx->accum.ensure(sky::job_accum_bytes(j)), "alloc partial sums", SKY_ERR_OOM);
-        j.accum = (float2 *)ctx->accum.p;
-        j.done = (uint32_t *)ctx->meta.p + 4;
-        j.t_done = (unsigned long long *)((uint32_t *)ctx->meta.p + 6);
-        HIP_TRY(hipMemcpyAsync(ctx->jobs.p, &j, sizeof(j), hipMemcpyHostToDevice, stream), "upload job",
+        j.accum = (float2 *)b.accum;
+        j.done = (uint32_t *)b.meta + 4;
+        j.t_done = (unsigned long long *)((uint32_t *)b.meta + 6);
+        ctx->host_jobs.assign(1, j);
+        HIP_TRY(hipMemcpyAsync(b.job, ctx->host_jobs.data(), sizeof(j), hipMemcpyHostToDevice, stream), "upload job",
                 SKY_ERR_COPY);
         plan = sky::plan_launch(ctx->tuning, ctx->num_cus, j.n_items, s.nlam * s.L, s.L, s.flags);
-    } else if (!s.film) {
-        size_t bytes = (size_t)npx * s.nlam * sizeof(float);
-        HIP_TRY(ctx->film.ensure(bytes), "alloc film", SKY_ERR_OOM);
-        s.film = (float *)ctx->film.p;
     }
     HIP_TRY(hipEventRecord(ctx->ev0, stream), "event record", SKY_ERR_SYNC);
     if (simple) {
         HIP_TRY(sky::launch_simple(s, stream, shape), "launch simple integrator", SKY_ERR_SYNC);
         if (s.xyz) HIP_TRY(sky::launch_film_to_xyz(s, stream), "launch xyz", SKY_ERR_SYNC);
     } else {
-        HIP_TRY(sky::launch_pool((const sky::DevJob *)ctx->jobs.p, 1, sky::finalize_blocks(ctx->host_jobs[0]), plan,
-                                 (uint32_t *)ctx->meta.p, 8, stream),
+        HIP_TRY(sky::launch_pool((const sky::DevJob *)b.job, 1, sky::finalize_blocks(j), plan, (uint32_t *)b.meta, 8,
+                                 stream),
                 "launch wave-pool integrator", SKY_ERR_SYNC);
         shape.grid = plan.grid;
         shape.block = plan.block;
@@ -150,8 +234,8 @@ sky_status run_render(sky_context *ctx, sky::HostScene &hs, hipStream_t stream,
     }
     HIP_TRY(hipEventRecord(ctx->ev1, stream), "event record", SKY_ERR_SYNC);
     uint32_t wd = 0;
-    HIP_TRY(hipMemcpyAsync(&wd, ctx->diag.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream),
-            "read diagnostics", SKY_ERR_COPY);
+    HIP_TRY(hipMemcpyAsync(&wd, b.diag, sizeof(uint32_t), hipMemcpyDeviceToHost, stream), "read diagnostics",
+            SKY_ERR_COPY);
     HIP_TRY(hipStreamSynchronize(stream), "synchronize", SKY_ERR_SYNC);
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
@@ -242,6 +326,10 @@ sky_status sky_context_create(int device, sky_context **out)
         (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess) {
         delete c;
         return hip_fail(e, "device initialisation", SKY_ERR_DEVICE_INIT);
+    }
+    if ((e = scratch_address(device, &c->scratch)) != hipSuccess) {
+        sky_context_destroy(c);
+        return hip_fail(e, "scratch address", SKY_ERR_DEVICE_INIT);
     }
     int cus = 0, khz = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
@@ -343,6 +431,28 @@ sky_status sky_render_device(sky_context *ctx, const sky_planet *planet, const s
     hs.dev.xyz = d_xyz;
     hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
     st = run_render(ctx, hs, s, stats);
+    if (stats)
+        stats->total_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return st;
+}
+
+sky_status sky_render_device_noalloc(sky_context *ctx, const sky_planet *planet, const sky_atmosphere *atm,
+                                     const sky_wavelengths *wl, const sky_render_config *cfg, float *d_film,
+                                     float *d_film_m2, float *d_xyz, void *stream, sky_render_stats *stats)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    sky::HostScene hs;
+    std::string err;
+    sky_status st = sky::build_scene(planet, atm, wl, cfg, hs, err);
+    if (st != SKY_OK) return fail(st, err);
+    if (!ctx) return fail(SKY_ERR_NO_DEVICE, "no device context");
+    if (!ctx->scratch) return fail(SKY_ERR_DEVICE_INIT, "context has no scratch address");
+    HIP_TRY(hipSetDevice(ctx->device), "set device", SKY_ERR_DEVICE_INIT);
+    hs.dev.film = d_film;
+    hs.dev.film_m2 = d_film_m2;
+    hs.dev.xyz = d_xyz;
+    st = run_render(ctx, hs, stream ? (hipStream_t)stream : ctx->stream, stats, ctx->scratch);
     if (stats)
         stats->total_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

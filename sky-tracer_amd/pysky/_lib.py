@@ -193,6 +193,8 @@ def lib() -> C.CDLL:
         "sky_render": ([ctxp, P, A, W, R, fp, fp, fp, C.POINTER(RenderStats)], C.c_int),
         "sky_render_device": ([ctxp, P, A, W, R, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                C.POINTER(RenderStats)], C.c_int),
+        "sky_render_device_noalloc": ([ctxp, P, A, W, R, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.POINTER(RenderStats)], C.c_int),
         "sky_optical_depth": ([ctxp, P, A, W, dp, C.c_uint32, dp], C.c_int),
         "sky_render_adjoint": ([ctxp, P, A, W, R, fp, dp, C.POINTER(RenderStats)], C.c_int),
         "sky_render_adjoint_device": ([ctxp, P, A, W, R, C.c_void_p, dp, C.c_void_p, C.POINTER(RenderStats)],
