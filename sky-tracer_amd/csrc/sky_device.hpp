@@ -310,10 +310,10 @@ __device__ __forceinline__ void ray_setup(const DevScene &s, const ShellTab &tab
     int k = 0;
     if constexpr (COUNT_L > 0) {
         // at most COUNT_L shells (L <= COUNT_L): the shell holding the origin
-        // is the number of interior boundaries at or below it
-#pragma unroll
-        for (int j = 1; j < COUNT_L; ++j)
-            if (j < L) k += tab[j] <= w0 ? 1 : 0;
+        // is the number of interior boundaries at or below it (L is the same
+        // for the whole wave, so the loop runs exactly L - 1 times)
+#pragma unroll 1
+        for (int j = 1; j < L; ++j) k += tab[j] <= w0 ? 1 : 0;
     } else {
         const float wf = (float)w0;
         const float h0 = f_div(wf, f_sqrt(fmaxf(s.R2f + wf, 0.0f)) + s.Rf);
