@@ -36,7 +36,7 @@ MODES = {
 # how oracle/Makefile builds the CPU baseline (gcc -O3 -march=x86-64-v3 measured no faster on
 # this path: 10.2 against 9.9 M samples/s on 8 threads)
 ORACLE_BUILD = "gcc -O2 -std=c11 -fopenmp (scalar, no -march), OpenMP dynamic schedule over pixels"
-PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 (packed vector FMA); unpacked vector issue is half
+PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
 # HBM bytes per integrator launch from rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (scales from
 # a 256 MiB copy): profiles/k_wavepool_c2_1024x512x64_counters.md.  The launch writes one
 # [pixel][band] float2 partial per pixel group and reads only the scene tables, so the figure
@@ -342,12 +342,18 @@ def run_fit(args, ctx):
     }
 
 
-def scan_flops_per_eval(primaries):
-    """Algorithmic FP64 operations of one (combo, sample) evaluation of the
-    reference's evaluate_combo (main.rs:443-458): per channel v * w / width and
-    three multiply-adds (8); XYZ / scale (3); XYZ -> LMS (15); three cube roots
-    (3); LMS' -> Lab (15); the distance (3 sub, 3 mul, 2 add, 1 sqrt = 9); the
-    two running sums (3).  Cube root and square root count as one each."""
+# FP32 flops of one (combo, sample) evaluation in k_scan<6>'s inner loop, counted from its gfx950
+# ISA (the loop is unrolled 4x): 23 FMAs (x2), 13 other FP32 adds/muls/subs and 7 transcendentals
+# (3 log2, 3 exp2, 1 sqrt; x1 each) per evaluation -- 57.4 VALU wave-instructions per wave64 of
+# evaluations by SQ_INSTS_VALU (profiles/k_scan_6band_combos_x_65536_samples_counters.md).
+SCAN_KERNEL_FLOPS_PER_EVAL = {6: 66}
+
+
+def scan_reference_ops_per_eval(primaries):
+    """The reference's FP64 operations per evaluation of evaluate_combo
+    (main.rs:443-458): per channel v * w / width and three multiply-adds
+    (8 per band); XYZ / scale (3); XYZ -> LMS (15); three cube roots (3);
+    LMS' -> Lab (15); the distance (9); the two running sums (3)."""
     return 8 * primaries + 3 + 15 + 3 + 15 + 9 + 3
 
 
@@ -416,8 +422,8 @@ def bench_scan(args):
     elapsed = time.perf_counter() - t0
     value = evals * args.steps / elapsed
     avg_s = sum(kernel_ms) / len(kernel_ms) / 1e3
-    f_eval = scan_flops_per_eval(P)
-    achieved = f_eval * evals / avg_s / 1e12
+    f_eval = SCAN_KERNEL_FLOPS_PER_EVAL.get(P)
+    achieved = f_eval * evals / avg_s / 1e12 if f_eval else None
     rmse = np.sqrt(tq.cpu().numpy() / idx.size)
     best = combos[int(np.argmin(rmse))]
     cpu = None
@@ -431,9 +437,13 @@ def bench_scan(args):
         "config": {"workload": f"scan: all {len(combos)} {P}-band combos of the 41-band grid, Voronoi weights, "
                                f"65536 of 655360 sky-view LUT samples ({args.spp or 64}-spp bake), one launch",
                    "parallelism": "1 GPU"},
-        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP32_TFLOPS, 5), "traffic": SCAN_TRAFFIC_BYTES.get(P),
-                     "flops_per_eval": f_eval, "kernel": f"k_scan<{P}>", "kernel_ms_avg": round(avg_s * 1e3, 2)},
+        "roofline": None if achieved is None else {
+            "bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP32_TFLOPS, 5), "traffic": SCAN_TRAFFIC_BYTES.get(P),
+            "flops_per_eval": f_eval, "reference_fp64_ops_per_eval": scan_reference_ops_per_eval(P),
+            "kernel": f"k_scan<{P}>", "kernel_ms_avg": round(avg_s * 1e3, 2)},
+        "precision": "FP32 per-sample colour (XYZ, cube roots, distance), FP64 sums over samples; "
+                     "sums within 2e-4 of the FP64 oracle (tests/test_gpu_scan.py)",
         "cpu_baseline": cpu,
         "best_screened_combo_nm": [float(wl[b]) for b in best],
     }
@@ -607,7 +617,6 @@ def main():
                         "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 5),
                         "traffic": TRAFFIC_BYTES.get(args.workload),
                         "flops_per_sample": round(f_unit, 1),
-                        "frac_nonpacked": round(achieved / (PEAK_FP32_TFLOPS / 2), 5),
                         "rng_int_tops": round(fm["configs"][key]["rng_int_ops_per_sample"] * samples
                                               / avg_kernel_s / 1e12, 3),
                         "kernel": pool_kernel(args.mode, params, w["n_lambda"]),

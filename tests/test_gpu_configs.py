@@ -163,3 +163,26 @@ def test_full_ragged_batch():
         scene, cfg = batch[i]
         single = pysky.render(scene, cfg, want_film=False, context=ctx)
         np.testing.assert_allclose(res.films[i].xyz, single.xyz, rtol=1e-4, atol=1e-6 * max(single.xyz.max(), 1e-30))
+
+
+def test_sky_view_sweep_batch_pairs_with_oracle():
+    """The reference's sky-view bake (scripts/bake_skyview_elevations.ps1:
+    ten sun elevations -2..90 deg, sky-view LUT projection, 41 bands of 10 nm
+    from 380 nm, uncapped depth with roulette from depth 4) as one batch of ten
+    renders at 32x32 and 1 spp: every item path for path with the oracle."""
+    lam = [380.0 + 10.0 * i for i in range(41)]
+    batch = []
+    for el in (-2.0, -1.0, 0.0, 1.0, 2.0, 5.0, 10.0, 20.0, 45.0, 90.0):
+        scene = pysky.SceneData(sun_elevation_deg=el, wavelengths_nm=lam, weights_nm=[10.0] * 41)
+        cfg = pysky.RenderConfig(width=32, height=32, spp=1, seed=0x5EC720260430, max_depth=0, rr_start_depth=4,
+                                 output_projection=pysky.OutputProjection.SkyViewLut, carried_tau=True,
+                                 phase_importance_sampling=True, majorant_layers=3, shell_growth=4.0)
+        batch.append((scene, cfg))
+    ctx = pysky.Context(0)
+    res = pysky.render_batch(batch, want_film=True, context=ctx)
+    for (scene, cfg), film in zip(batch, res.films):
+        ref, _, _, _, rc = O.render(oracle_params(scene, cfg), "f32")
+        assert rc == 0
+        frac, _ = paired_stats(film.values(), ref.ravel())
+        print(f"[gpu] sky-view sweep, sun {scene.sun_elevation_deg:+.0f} deg: {frac:.4%} paired", flush=True)
+        assert frac >= 0.99
