@@ -38,10 +38,10 @@ MODES = {
 ORACLE_BUILD = "gcc -O2 -std=c11 -fopenmp (scalar, no -march), OpenMP dynamic schedule over pixels"
 PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
 # HBM bytes per integrator launch from rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (scales from
-# a 256 MiB copy): profiles/k_wavepool_c2_1024x512x64_counters.md.  The launch writes one
-# [pixel][band] float2 partial per pixel group and reads only the scene tables, so the figure
-# does not depend on spp at the tuned work split.
-TRAFFIC_BYTES = {"c2": (2.0 * 219.3 + 276672.1) * 1024}
+# a 256 MiB copy): profiles/k_wavepool_c2_1024x512x64_counters.md.  Each work item writes its
+# pixels' XYZ (the 6.3 MB image, as partial cache lines shared with items on other XCDs) and
+# reads only the scene tables, so the figure does not depend on spp at the tuned work split.
+TRAFFIC_BYTES = {"c2": (2.0 * 1116.12 + 24256.0) * 1024}
 # Same for the combo scan of all 6-band combos x 65536 samples:
 # profiles/k_scan_6band_combos_x_65536_samples_counters.md.
 SCAN_TRAFFIC_BYTES = {6: (2.0 * 5189454.0 + 70256.1) * 1024}
