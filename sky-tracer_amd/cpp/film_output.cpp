@@ -1,6 +1,8 @@
 // film_output.cpp -- see film_output.hpp.
 #include "film_output.hpp"
 
+#include <zlib.h>
+
 #include <algorithm>
 #include <cerrno>
 #include <cmath>
@@ -268,6 +270,48 @@ void write_exr_rgb(const std::string &path, uint32_t width, uint32_t height, con
     if (std::fwrite(b.data(), 1, b.size(), f.f) != b.size()) throw std::runtime_error("cannot write " + path);
 }
 
+// OpenEXR chunk decoding (file-layout document, "Compression"): RLE and ZIP
+// streams hold the block's bytes after two reversible steps -- the bytes split
+// into even and odd positions (first half, second half), then each byte
+// replaced by its difference to the previous one plus 128.  RLE: a signed
+// count byte c; c < 0 copies -c literal bytes, c >= 0 repeats the next byte
+// c + 1 times.  ZIP/ZIPS: a zlib stream.  A chunk whose stored size equals the
+// uncompressed size is stored raw.
+namespace {
+
+bool rle_decode(const uint8_t *in, size_t n, std::vector<uint8_t> &out, size_t want)
+{
+    out.clear();
+    size_t i = 0;
+    while (i < n) {
+        const int c = (int8_t)in[i++];
+        if (c < 0) {
+            const size_t k = (size_t)(-c);
+            if (i + k > n || out.size() + k > want) return false;
+            out.insert(out.end(), in + i, in + i + k);
+            i += k;
+        } else {
+            if (i >= n || out.size() + (size_t)c + 1 > want) return false;
+            out.insert(out.end(), (size_t)c + 1, in[i++]);
+        }
+    }
+    return out.size() == want;
+}
+
+void unpredict_deinterleave(std::vector<uint8_t> &t, std::vector<uint8_t> &out)
+{
+    const size_t n = t.size();
+    for (size_t i = 1; i < n; ++i) t[i] = (uint8_t)(t[i - 1] + t[i] - 128);
+    out.resize(n);
+    const size_t half = (n + 1) / 2;
+    for (size_t i = 0; i < half; ++i) {
+        out[2 * i] = t[i];
+        if (2 * i + 1 < n) out[2 * i + 1] = t[half + i];
+    }
+}
+
+}  // namespace
+
 std::vector<float> read_exr_r(const std::string &path, uint32_t &width, uint32_t &height)
 {
     File f(path, "rb");
@@ -275,9 +319,11 @@ std::vector<float> read_exr_r(const std::string &path, uint32_t &width, uint32_t
     uint8_t buf[65536];
     size_t n;
     while ((n = std::fread(buf, 1, sizeof buf, f.f)) > 0) b.insert(b.end(), buf, buf + n);
-    auto fail = [&](const char *why) { return std::runtime_error("EXR " + path + ": " + why); };
+    auto fail = [&](const std::string &why) { return std::runtime_error("EXR " + path + ": " + why); };
     if (b.size() < 8 || b[0] != 0x76 || b[1] != 0x2f || b[2] != 0x31 || b[3] != 0x01) throw fail("not an EXR file");
-    if ((b[4] != 2) || (b[5] & 0x16)) throw fail("only single-part scanline files are read");
+    // version 2; flags: 0x200 tiled, 0x400 long names, 0x800 deep data, 0x1000 multi-part
+    if (b[4] != 2 || (b[5] & 0x18)) throw fail("only single-part, non-deep files are read");
+    const bool tiled = (b[5] & 0x02) != 0;
     size_t p = 8;
     auto cstr = [&]() {
         size_t s = p;
@@ -286,7 +332,7 @@ std::vector<float> read_exr_r(const std::string &path, uint32_t &width, uint32_t
         return std::string((const char *)&b[s], (p++) - s);
     };
     auto i32 = [&](size_t at) {
-        if (at + 4 > b.size()) throw fail("truncated header");
+        if (at + 4 > b.size()) throw fail("truncated file");
         int32_t v;
         std::memcpy(&v, &b[at], 4);
         return v;
@@ -298,6 +344,7 @@ std::vector<float> read_exr_r(const std::string &path, uint32_t &width, uint32_t
     std::vector<Ch> chans;
     int compression = -1;
     int32_t xmin = 0, ymin = 0, xmax = -1, ymax = -1;
+    uint32_t tile_w = 0, tile_h = 0;
     for (;;) {
         if (p >= b.size()) throw fail("truncated header");
         if (b[p] == 0) {
@@ -321,6 +368,10 @@ std::vector<float> read_exr_r(const std::string &path, uint32_t &width, uint32_t
                 int32_t t;
                 std::memcpy(&t, &b[q], 4);
                 if (t < 0 || t > 2) throw fail("unknown channel type");
+                int32_t xs, ys;
+                std::memcpy(&xs, &b[q + 8], 4);
+                std::memcpy(&ys, &b[q + 12], 4);
+                if (xs != 1 || ys != 1) throw fail("subsampled channels are not read");
                 chans.push_back({cn, t});
                 q += 16;
             }
@@ -333,15 +384,21 @@ std::vector<float> read_exr_r(const std::string &path, uint32_t &width, uint32_t
             ymin = i32(p + 4);
             xmax = i32(p + 8);
             ymax = i32(p + 12);
+        } else if (name == "tiles") {
+            if (size < 9) throw fail("truncated attribute");
+            std::memcpy(&tile_w, &b[p], 4);
+            std::memcpy(&tile_h, &b[p + 4], 4);
         }
         p = end;
     }
-    if (compression != 0) throw fail("only uncompressed files are read");
+    if (compression < 0 || compression > 3)
+        throw fail("compression " + std::to_string(compression) + " is not read (NONE, RLE, ZIPS and ZIP are)");
     if (xmax < xmin || ymax < ymin || (int64_t)xmax - xmin >= (1 << 24) || (int64_t)ymax - ymin >= (1 << 24))
         throw fail("bad data window");
     width = (uint32_t)(xmax - xmin + 1);
     height = (uint32_t)(ymax - ymin + 1);
-    if (p + 8 * (size_t)height > b.size()) throw fail("truncated offset table");
+    if (tiled && (tile_w == 0 || tile_h == 0 || tile_w > (1u << 16) || tile_h > (1u << 16)))
+        throw fail("bad tile description");
     size_t pixel_bytes = 0, r_off = (size_t)-1;
     int r_type = -1;
     for (const Ch &c : chans) {
@@ -353,22 +410,67 @@ std::vector<float> read_exr_r(const std::string &path, uint32_t &width, uint32_t
     }
     if (r_off == (size_t)-1) throw fail("no R channel");
     if (r_type == 0) throw fail("R channel is UINT");
+    // blocks: scanline chunks of 1 (NONE, RLE, ZIPS) or 16 (ZIP) lines, or level-0 tiles
+    const uint32_t lines_per_block = compression == 3 ? 16u : 1u;
+    const uint32_t bw = tiled ? tile_w : width, bh = tiled ? tile_h : lines_per_block;
+    const uint32_t nbx = (width + bw - 1) / bw, nby = (height + bh - 1) / bh;
+    const size_t nblocks = (size_t)nbx * nby;
+    if (p + 8 * nblocks > b.size()) throw fail("truncated offset table");
     std::vector<float> out((size_t)width * height);
-    for (uint32_t y = 0; y < height; ++y) {
+    std::vector<uint8_t> raw, tmp;
+    for (size_t k = 0; k < nblocks; ++k) {
         uint64_t off;
-        std::memcpy(&off, &b[p + 8 * (size_t)y], 8);
-        if (off > b.size() || off + 8 + pixel_bytes * width > b.size()) throw fail("truncated scanline");
-        size_t line = (size_t)off + 8 + r_off * width;  // channel planes of one line
-        for (uint32_t x = 0; x < width; ++x) {
-            float v;
-            if (r_type == 1) {
-                uint16_t h;
-                std::memcpy(&h, &b[line + 2 * x], 2);
-                v = half_to_float(h);
+        std::memcpy(&off, &b[p + 8 * k], 8);
+        if (off >= b.size()) throw fail("offset out of range");
+        uint32_t bx, by;
+        size_t data;
+        if (tiled) {
+            const int32_t tx = i32(off), ty = i32(off + 4), lx = i32(off + 8), ly = i32(off + 12);
+            if (lx != 0 || ly != 0) continue;  // not the full-resolution level
+            if (tx < 0 || ty < 0 || (uint32_t)tx >= nbx || (uint32_t)ty >= nby) throw fail("bad tile coordinates");
+            bx = (uint32_t)tx;
+            by = (uint32_t)ty;
+            data = (size_t)off + 16;
+        } else {
+            const int32_t y = i32(off);
+            if (y < ymin || y > ymax || (uint32_t)(y - ymin) % bh) throw fail("bad scanline block");
+            bx = 0;
+            by = (uint32_t)(y - ymin) / bh;
+            data = (size_t)off + 4;
+        }
+        const int32_t packed = i32(data);
+        data += 4;
+        if (packed < 0 || data + (size_t)packed > b.size()) throw fail("truncated block");
+        const uint32_t x0 = bx * bw, y0 = by * bh;
+        const uint32_t cw = std::min(bw, width - x0), chh = std::min(bh, height - y0);
+        const size_t want = pixel_bytes * cw * chh;
+        const uint8_t *src = &b[data];
+        if ((size_t)packed != want) {
+            if (compression == 0) throw fail("block size does not match the data window");
+            if (compression == 1) {
+                if (!rle_decode(src, (size_t)packed, tmp, want)) throw fail("bad RLE block");
             } else {
-                std::memcpy(&v, &b[line + 4 * x], 4);
+                tmp.resize(want);
+                uLongf dst = (uLongf)want;
+                if (uncompress(tmp.data(), &dst, src, (uLong)packed) != Z_OK || dst != want)
+                    throw fail("bad ZIP block");
             }
-            out[(size_t)y * width + x] = v;
+            unpredict_deinterleave(tmp, raw);
+            src = raw.data();
+        }
+        // per line of the block: every channel's cw samples in channel order
+        for (uint32_t yy = 0; yy < chh; ++yy) {
+            const uint8_t *line = src + (size_t)yy * pixel_bytes * cw + r_off * cw;
+            float *dst = &out[(size_t)(y0 + yy) * width + x0];
+            for (uint32_t x = 0; x < cw; ++x) {
+                if (r_type == 1) {
+                    uint16_t h;
+                    std::memcpy(&h, line + 2 * x, 2);
+                    dst[x] = half_to_float(h);
+                } else {
+                    std::memcpy(&dst[x], line + 4 * x, 4);
+                }
+            }
         }
     }
     return out;

@@ -46,8 +46,11 @@ bool update_asset_colorimetry(const std::string &out_dir, const SceneData &scene
 
 // Single-part scanline EXR with float B, G, R channels ([pixel][3] = r, g, b).
 void write_exr_rgb(const std::string &path, uint32_t width, uint32_t height, const std::vector<float> &rgb);
-// Reads the R channel of a file written by write_exr_rgb (or any uncompressed
-// float/half scanline EXR with an R channel).
+// Reads the R channel of a single-part EXR: scanline or tiled (full-resolution
+// level), compression NONE, RLE, ZIPS or ZIP, R stored as half or float.  The
+// band images other tools write (e.g. RLE-compressed 64x64 tiles) read the same
+// as write_exr_rgb's uncompressed scanlines.  Other compressions, deep and
+// multi-part files raise std::runtime_error.
 std::vector<float> read_exr_r(const std::string &path, uint32_t &width, uint32_t &height);
 void write_png_rgb8(const std::string &path, uint32_t width, uint32_t height, const std::vector<uint8_t> &rgb);
 

@@ -10,6 +10,7 @@ independent readers (tests/exr_png.py) and the PNG's CRCs and Adler-32 are
 verified by zlib."""
 import json
 import os
+import struct
 import subprocess
 
 import numpy as np
@@ -43,11 +44,11 @@ def _rebuild(out, *args):
     return _run("--rebuild-rgb-only", "--bands", "16", "--out", str(out), *args)
 
 
-def _write_bands(out, film, half=False):
+def _write_bands(out, film, **encoding):
     os.makedirs(out / "bands", exist_ok=True)
     for b, nm in enumerate(_bands(film.shape[2])):
         v = film[:, :, b]
-        write_exr(out / "bands" / _band_name(nm), {"R": v, "G": v, "B": v}, half=half)
+        write_exr(out / "bands" / _band_name(nm), {"R": v, "G": v, "B": v}, **encoding)
 
 
 def _film(h=12, w=20, nb=16, seed=5):
@@ -135,6 +136,65 @@ def test_rebuild_reads_half_float_bands(tmp_path):
     exr = read_exr(tmp_path / "sky_rgb.exr")
     rgb_want, _, _, _ = _expected(film, lam)
     np.testing.assert_allclose(exr["G"], rgb_want[..., 1], rtol=2e-5, atol=1e-6 * np.abs(rgb_want).max())
+
+
+ENCODINGS = [
+    dict(compression="rle", tile=(64, 64)),  # what exr 1.74's write_rgb_file emits (RLE, 64x64 tiles)
+    dict(compression="rle"),
+    dict(compression="zips"),
+    dict(compression="zip"),
+    dict(compression="zip", tile=(8, 5), shuffle=3),
+    dict(compression="rle", tile=(7, 4), mipmap=True),
+    dict(compression="none", tile=(6, 6)),
+    dict(compression="zips", half=True),
+]
+
+
+@pytest.mark.parametrize("encoding", ENCODINGS, ids=lambda e: "-".join(f"{k}={v}" for k, v in e.items()))
+def test_rebuild_reads_compressed_and_tiled_bands(tmp_path, encoding):
+    """Band EXRs written as RLE/ZIPS/ZIP scanlines or tiles (ragged edge
+    tiles, chunks out of order, a mipmap chain whose smaller levels must be
+    skipped) rebuild the same bytes as the uncompressed scanline bands.
+    37 lines leave a ragged 5-line ZIP block; zero and constant regions give
+    the run-length coder runs."""
+    film, _ = _film(h=37, w=23)
+    film[5:9] = 0.0
+    film[12, :, :] = 1.0
+    half = encoding.get("half", False)
+    if half:
+        film = film.astype(np.float16).astype(np.float32)
+    plain, coded = tmp_path / "plain", tmp_path / "coded"
+    _write_bands(plain, film, half=half)
+    _write_bands(coded, film, **encoding)
+    size = lambda d: sum(f.stat().st_size for f in (d / "bands").iterdir())
+    if encoding["compression"] != "none" and not encoding.get("mipmap"):
+        assert size(coded) < size(plain)  # some chunks are stored compressed
+    for d in (plain, coded):
+        r = _rebuild(d)
+        assert r.returncode == 0, r.stderr
+    for name in ("sky_rgb.exr", "sky_rgb.png"):
+        assert (coded / name).read_bytes() == (plain / name).read_bytes(), name
+
+
+def _patched(path, old, new):
+    b = path.read_bytes()
+    assert b.count(old) == 1
+    path.write_bytes(b.replace(old, new))
+
+
+def test_rebuild_rejects_unread_encodings(tmp_path):
+    film, _ = _film(h=4, w=6)
+    _write_bands(tmp_path, film)
+    band = tmp_path / "bands" / _band_name(_bands()[2])
+    # PIZ (4) is named and refused, not misread
+    _patched(band, b"compression\0compression\0\x01\0\0\0\0", b"compression\0compression\0\x01\0\0\0\x04")
+    r = _rebuild(tmp_path)
+    assert r.returncode == 1 and "compression 4 is not read" in r.stderr and band.name in r.stderr
+    _write_bands(tmp_path, film)
+    # multi-part flag (0x1000 in the version field)
+    _patched(band, struct.pack("<ii", 20000630, 2), struct.pack("<ii", 20000630, 0x1002))
+    r = _rebuild(tmp_path)
+    assert r.returncode == 1 and "single-part" in r.stderr
 
 
 def test_rebuild_png_exposure(tmp_path):
