@@ -50,7 +50,9 @@ struct DevScene {
     float Rf, R2f, topf, eps;
     float sun[3];
     float omc, cone_pdf;  // 1 - cos(sun radius) (= 2 sin^2(r/2)), 1/(2 pi omc)
+    float cone_omega;     // 2 pi omc: the sun cone's solid angle, 1 / cone_pdf
     float inv_hr, inv_ha, hc, inv_hw, g, albedo;
+    float inv_2g, inv_dls;  // 1 / (2 g) (0 when |g| < 1e-3), 1 / direct_light_samples
     float l2_hr, l2_ha;   // log2(e) / H: exp(-h / H) = exp2(-h l2_h)
     float inv_w, inv_h;   // 1 / width, 1 / height
     // Sky-view LUT frame (sky_trace.wgsl:641-681), precomputed on the host.
@@ -100,15 +102,16 @@ __device__ __forceinline__ float free_flight(float u)
     return -kLn2 * f_log2(fmaxf(1.0f - u, 0.0000001f));
 }
 
-// sqrt(x) for x >= 0 in FP64: FP32 root, one Newton step with an FP32
-// reciprocal; relative error ~1e-14.
+// sqrt(x) for x >= 0 in FP64: the FP32 reciprocal root r gives the root
+// x r and, halved, the Newton step's 1 / (2 sqrt x); relative error ~1e-13
+// (0 for x below 1e-30).
 __device__ __forceinline__ double sqrt64(double x)
 {
-    float sf = f_sqrt((float)x);
-    if (!(sf > 0.0f)) return 0.0;
-    double s = (double)sf;
-    double r = (double)(0.5f * f_rcp(sf));
-    return s + (x - s * s) * r;
+    const float xf = (float)x;
+    if (!(xf > 1e-30f)) return 0.0;
+    const float r = f_rsq(xf);
+    const double s = (double)(xf * r);
+    return s + (x - s * s) * (double)(0.5f * r);
 }
 
 // 1 / |d| in FP64 for an FP32-normalised d (|d|^2 - 1 ~ 1e-7): third-order
@@ -119,26 +122,12 @@ __device__ __forceinline__ double inv_norm64(double dx, double dy, double dz)
     return 1.0 + e * (-0.5 + e * (0.375 - 0.3125 * e));
 }
 
-// Quotient n / d for n < 2^24 from the float reciprocal rd = 1 / d (correctly
-// rounded): the float quotient is within 2 of the true one.
-__device__ __forceinline__ uint32_t udiv24(uint32_t n, uint32_t d, float rd)
+// floor(n / d) for n < 2^21 from rd = 1 / d (within an ulp): (n + 1/2) / d
+// lies at least 1/(2d) from an integer, and the float product's error is
+// below (n + 1/2) / d * 2^-22 < 1/(2d).
+__device__ __forceinline__ uint32_t udiv_small(uint32_t n, float rd)
 {
-    uint32_t q = (uint32_t)((float)n * rd);
-    int32_t r = (int32_t)(n - q * d);
-    if (r < 0) {
-        q--;
-        r += (int32_t)d;
-        if (r < 0) {
-            q--;
-            r += (int32_t)d;
-        }
-    }
-    if (r >= (int32_t)d) {
-        q++;
-        r -= (int32_t)d;
-        if (r >= (int32_t)d) q++;
-    }
-    return q;
+    return (uint32_t)(((float)n + 0.5f) * rd);
 }
 
 // ---------------------------------------------------------------- RNG
@@ -463,8 +452,10 @@ __device__ __forceinline__ float phase_rayleigh(float mu)
 
 __device__ __forceinline__ float phase_hg(float g, float mu)
 {
+    // (1 - g^2) / (4 pi den^1.5), den^-1.5 as the cube of one reciprocal root
     float den = 1.0f + g * g - 2.0f * g * mu;
-    return (1.0f - g * g) * (0.25f * kInvPi) * f_rsq(den) * f_rcp(den);
+    float r = f_rsq(den);
+    return (1.0f - g * g) * (0.25f * kInvPi) * (r * r * r);
 }
 
 // mixed_phase_value, sky_trace.wgsl:496-510 (Rayleigh + HG aerosol).
@@ -488,11 +479,12 @@ __device__ __forceinline__ float sample_rayleigh_mu(float xi)
     return fminf(fmaxf(mu, -1.0f), 1.0f);
 }
 
-__device__ __forceinline__ float sample_hg_mu(float g, float xi)
+// inv_2g = 1 / (2 g), precomputed on the host.
+__device__ __forceinline__ float sample_hg_mu(float g, float inv_2g, float xi)
 {
     if (fabsf(g) < 1e-3f) return 1.0f - 2.0f * xi;
     float sv = f_div(1.0f - g * g, 1.0f - g + 2.0f * g * xi);
-    float mu = f_div(1.0f + g * g - sv * sv, 2.0f * g);
+    float mu = (1.0f + g * g - sv * sv) * inv_2g;
     return fminf(fmaxf(mu, -1.0f), 1.0f);
 }
 
@@ -514,12 +506,13 @@ __device__ __forceinline__ f3 sun_cone_dir(const DevScene &s, float xi0, float x
 }
 
 // direction_in_cone (sky_trace.wgsl:562-564) as |d - s|^2 <= 2 (1 - cos r).
-__device__ __forceinline__ bool in_sun_cone(const DevScene &s, f3 d)
+__device__ __forceinline__ bool in_sun_cone_unit(const DevScene &s, f3 n)
 {
-    f3 n = normalize_or_y(d);
     float ax = n.x - s.sun[0], ay = n.y - s.sun[1], az = n.z - s.sun[2];
     return ax * ax + ay * ay + az * az <= 2.0f * s.omc;
 }
+
+__device__ __forceinline__ bool in_sun_cone(const DevScene &s, f3 d) { return in_sun_cone_unit(s, normalize_or_y(d)); }
 
 // Camera ray direction (sky_trace.wgsl:614-689).
 __device__ __forceinline__ f3 camera_dir(const DevScene &s, uint32_t x, uint32_t y, float j1,

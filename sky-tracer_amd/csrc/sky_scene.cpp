@@ -549,6 +549,7 @@ sky_status build_scene(const sky_planet *planet, const sky_atmosphere *a, const 
     double omc = 2.0 * hsin * hsin;
     s.omc = (float)omc;
     s.cone_pdf = (float)(1.0 / (2.0 * kPiD * omc));
+    s.cone_omega = (float)(2.0 * kPiD * omc);
     s.inv_hr = (float)(1.0 / a->rayleigh_h_km);
     s.inv_ha = (float)(1.0 / a->aerosol_h_km);
     s.l2_hr = (float)(1.4426950408889634 / a->rayleigh_h_km);
@@ -556,6 +557,7 @@ sky_status build_scene(const sky_planet *planet, const sky_atmosphere *a, const 
     s.hc = (float)a->ozone_center_km;
     s.inv_hw = (float)(1.0 / a->ozone_half_width_km);
     s.g = (float)a->aerosol_g;
+    s.inv_2g = std::fabs(s.g) < 1e-3f ? 0.0f : 1.0f / (2.0f * s.g);
     s.albedo = (float)a->ground_albedo;
     // Sky-view LUT frame, sky_trace.wgsl:632-681
     {
@@ -589,6 +591,7 @@ sky_status build_scene(const sky_planet *planet, const sky_atmosphere *a, const 
     s.seed_hi = (uint32_t)(cfg->seed >> 32);
     s.nlam = wl->count;
     s.dls = cfg->direct_light_samples ? cfg->direct_light_samples : 1u;  // integrator.rs:325
+    s.inv_dls = 1.0f / (float)s.dls;
     s.projection = cfg->projection;
     s.max_depth = cfg->max_depth;
     s.rr_start = cfg->rr_start_depth;

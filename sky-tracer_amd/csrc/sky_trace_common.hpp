@@ -159,7 +159,7 @@ __device__ float trace_path(const DevScene &s, uint32_t x, uint32_t y, uint32_t 
             if (s.flags & kFlagPhaseIS) {
                 float xl = rng_next(rng);
                 float xm = rng_next(rng);
-                float mu = xl * scat < co.sr ? sample_rayleigh_mu(xm) : sample_hg_mu(s.g, xm);
+                float mu = xl * scat < co.sr ? sample_rayleigh_mu(xm) : sample_hg_mu(s.g, s.inv_2g, xm);
                 float st = f_sqrt(fmaxf(1.0f - mu * mu, 0.0f));
                 nd = frame_dir(ray.d, mu, st, rng_next(rng));
                 w = 1.0f;

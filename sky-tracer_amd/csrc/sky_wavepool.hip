@@ -384,9 +384,13 @@ __device__ __forceinline__ void track_step_fast(const DevScene &s, const MajSrc 
     float tn = 0.0f;
     if (c.tau < 0.0f && c.t < c.seg_end) c.tau = free_flight(rng_next(c.rng));
     bool ended = false;
+    // 1 / majorant of the current segment; the loop leaves with the segment
+    // of the collision or of the ray's end, whose ratio below reuses it
+    float rm;
     for (;;) {
         const bool nonempty = c.t < c.seg_end;
-        tn = c.t + f_div(c.tau, c.maj);
+        rm = f_rcp(c.maj);
+        tn = c.t + c.tau * rm;
         if constexpr (CARRIED) {
             const float seg_tau = c.maj * (c.seg_end - c.t);
             const bool cross = !nonempty || c.tau >= seg_tau;
@@ -421,7 +425,7 @@ __device__ __forceinline__ void track_step_fast(const DevScene &s, const MajSrc 
     const float h = lane_altitude(s, c, col ? tn : c.t);
     const Coeffs co = lane_medium<true>(s, m, c, h);
     const float st = co.sr + co.sas + co.so + co.saa;
-    const float ratio = fminf(fmaxf(f_div(st, c.maj), 0.0f), 1.0f);
+    const float ratio = fminf(fmaxf(st * rm, 0.0f), 1.0f);
     const float u = rng_peek(c.rng, 0);
     c.rng.w += col && !rt ? kLcgMul : 0u;
     const bool acc = col && !rt && u < ratio;
@@ -583,7 +587,7 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
             phi = u1;
         } else if (c.vtype == 0) {
             if (phase_is) {
-                mu = u0 < c.rw ? sample_rayleigh_mu(u1) : sample_hg_mu(s.g, u1);
+                mu = u0 < c.rw ? sample_rayleigh_mu(u1) : sample_hg_mu(s.g, s.inv_2g, u1);
                 axis = c.vin;
                 phi = u2;
             } else {
@@ -721,16 +725,18 @@ __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSr
     const bool bnd = c.ev == EV_BOUNDARY, hit = c.ev == EV_HIT, shd = !bnd && !hit;
     const bool at_ground = bnd && c.kind == 0;
     {
-        // advance_vertex for a ground hit or an accepted collision
+        // advance_vertex for a ground hit or an accepted collision (a lane
+        // that does not move adds d * 0, which leaves its position exact)
         const double dx = c.d.x, dy = c.d.y, dz = c.d.z;
         const double f = inv_norm64(dx, dy, dz) * (double)(at_ground ? c.tmax : c.t);
-        const bool mv = at_ground || hit;
-        c.px = mv ? c.px + dx * f : c.px;
-        c.py = mv ? c.py + dy * f : c.py;
-        c.pz = mv ? c.pz + dz * f : c.pz;
+        const double fm = at_ground || hit ? f : 0.0;
+        c.px += dx * fm;
+        c.py += dy * fm;
+        c.pz += dz * fm;
     }
-    // exit: the sun disk is seen only by the camera ray (wgsl:745-749)
-    const bool sun_seen = bnd && c.kind == 1 && c.depth == 0 && in_sun_cone(s, c.d);
+    // exit: the sun disk is seen only by the camera ray (wgsl:745-749); the
+    // ray directions are unit vectors already (camera_dir, frame_dir)
+    const bool sun_seen = bnd && c.kind == 1 && c.depth == 0 && in_sun_cone_unit(s, c.d);
     if constexpr (ADJ) {
         // the sun disk's contribution with the path's score, a sun sample's
         // with the sample's score
@@ -775,12 +781,12 @@ __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSr
     //    (sample_isotropic, wgsl:534-539, as a frame around +y with the
     //    azimuth negated) at a scattering vertex, cosine lobe at the ground
     const f3 nrm = normalize_or_y(f3{(float)c.px, (float)c.py, (float)c.pz});
-    float mu_sun, st_sun;
-    sun_cone_mu(s, u0, mu_sun, st_sun);
+    // sun cone (sun_cone_mu): mu = 1 - x, sin = sqrt(x (2 - x))
+    const float x_sun = fminf(u0, 0.99999994f) * s.omc;
     float mu_v0, phi_v;
     f3 axis_v;
     if constexpr (IS) {
-        mu_v0 = u0 < c.rw ? sample_rayleigh_mu(u1) : sample_hg_mu(s.g, u1);
+        mu_v0 = u0 < c.rw ? sample_rayleigh_mu(u1) : sample_hg_mu(s.g, s.inv_2g, u1);
         axis_v = c.vin;
         phi_v = u2;
     } else {
@@ -788,10 +794,10 @@ __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSr
         axis_v = f3{0.0f, 1.0f, 0.0f};
         phi_v = -u1;
     }
-    const float st_v = f_sqrt(fmaxf(1.0f - mu_v0 * mu_v0, 0.0f));
-    const float mu_cos = f_sqrt(fmaxf(1.0f - u0, 0.0f)), st_cos = f_sqrt(u0);
-    const float mu = is_nee ? mu_sun : (scat_v ? mu_v0 : mu_cos);
-    const float st = is_nee ? st_sun : (scat_v ? st_v : st_cos);
+    const float mu_cos = f_sqrt(fmaxf(1.0f - u0, 0.0f));
+    const float mu = is_nee ? 1.0f - x_sun : (scat_v ? mu_v0 : mu_cos);
+    // one root for the three polar sines: x (2 - x), 1 - mu^2 (clamped), u0
+    const float st = f_sqrt(is_nee ? x_sun * (2.0f - x_sun) : (scat_v ? fmaxf(1.0f - mu_v0 * mu_v0, 0.0f) : u0));
     const float phi = (!is_nee && scat_v) ? phi_v : u1;
     const f3 sun{s.sun[0], s.sun[1], s.sun[2]};
     const f3 axis = is_nee ? sun : (scat_v ? axis_v : (c.vtype == 1 ? nrm : f3{0.0f, 1.0f, 0.0f}));
@@ -803,11 +809,11 @@ __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSr
     const float cs = fmaxf(dot3(gn, dir), 0.0f);
     const float mu_v = dot3(c.vin, dir);
     const float ph = c.rw * phase_rayleigh(mu_v) + (1.0f - c.rw) * phase_hg(s.g, mu_v);
-    const float factor = f_div(c.vtype == 1 ? s.albedo * kInvPi * lsun * cs : lsun * ph, s.cone_pdf);
+    const float factor = (c.vtype == 1 ? s.albedo * kInvPi * lsun * cs : lsun * ph) * s.cone_omega;
     const bool nee_ray = c.vtype == 1 ? cs > 0.0f : true;
     // CONT: the NEE average, the bounce weight (1 for lobe sampling, albedo
     // for the cosine lobe) and the roulette (wgsl:691-701, 751-770)
-    const float Lc = c.L + c.tp * f_div(c.nee_sum, (float)n_light);
+    const float Lc = c.L + c.tp * (c.nee_sum * s.inv_dls);
     const bool ok = scat_v || cs > 0.0f;
     // bounce weight: 1 for a lobe sample, phase / isotropic pdf for an
     // isotropic one, the albedo for the cosine lobe
@@ -839,9 +845,12 @@ __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSr
         c.tp = ok ? (killed ? tp1 : tp2) : c.tp;
     }
     c.depth = bounce ? c.depth + 1u : c.depth;
-    c.px = bounce ? c.px + (double)dir.x * kRayEpsilonKm : c.px;
-    c.py = bounce ? c.py + (double)dir.y * kRayEpsilonKm : c.py;
-    c.pz = bounce ? c.pz + (double)dir.z * kRayEpsilonKm : c.pz;
+    {
+        const double e = bounce ? (double)kRayEpsilonKm : 0.0;
+        c.px += (double)dir.x * e;
+        c.py += (double)dir.y * e;
+        c.pz += (double)dir.z * e;
+    }
     nx = is_cont ? (bounce ? NX_MAIN : NX_FINISH) : nx;
     // loop top of trace_path (wgsl:709-717) plus the depth cap
     bool main_ray = false;
@@ -900,15 +909,14 @@ struct ItemInfo {
     float r_nslots, r_nlam, r_width;
 };
 
-// Path start (main, sky_trace.wgsl:799-807): triple j of the item, ordered
-// (sample, pixel, band) with the band fastest.
+// Path start (main, sky_trace.wgsl:799-807): triple j = si * nslots + rem of
+// the item, ordered (sample, pixel, band) with the band fastest.
 template <bool ADJ, bool FAST>
 __device__ __forceinline__ void start_path(const DevScene &s, LaneCtx &c, AdjLane &a, const AdjArgs &A,
-                                           uint32_t j, const ItemInfo &it)
+                                           uint32_t si, uint32_t rem, const ItemInfo &it)
 {
-    uint32_t si = udiv24(j, it.nslots, it.r_nslots);
-    uint32_t rem = j - si * it.nslots;
-    uint32_t pl = udiv24(rem, s.nlam, it.r_nlam);
+    // rem < nslots <= 256
+    uint32_t pl = udiv_small(rem, it.r_nlam);
     c.band = rem - pl * s.nlam;
     c.slot = rem;
     // pixel p0 + pl, from the item's first pixel (x0, y0); pl < 256
@@ -919,7 +927,7 @@ __device__ __forceinline__ void start_path(const DevScene &s, LaneCtx &c, AdjLan
             y++;
         }
     } else {
-        uint32_t dy = udiv24(x, s.width, it.r_width);
+        uint32_t dy = udiv_small(x, it.r_width);
         x -= dy * s.width;
         y += dy;
     }
@@ -1028,7 +1036,8 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
     constexpr bool FAST = FK != 0;
     extern __shared__ float dyn_lds[];
     uint32_t loaded = 0xFFFFFFFFu;  // job whose majorants are in the wave's LDS copy
-    const uint32_t wid = threadIdx.x >> 6;
+    // wave index, read into a scalar register: the wave's LDS bases are uniform
+    const uint32_t wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t lane = lane_id();
     float *wave_lds = dyn_lds + wid * (kLdsFixed + a.maj_lds + 2 * a.shell_lds + a.lam_lds);
     float *acc_sum = wave_lds;
@@ -1067,7 +1076,6 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
         it.r_nslots = 1.0f / (float)it.nslots;
         it.r_nlam = 1.0f / (float)s.nlam;
         it.r_width = 1.0f / (float)s.width;
-        const uint32_t total = it.nslots * ns;
         for (uint32_t i = lane; i < it.nslots; i += 64) {
             acc_sum[i] = 0.0f;
             acc_sq[i] = 0.0f;
@@ -1101,12 +1109,13 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        uint32_t next = 0;
+        // next triple to hand out, next_si * nslots + next_rem (wave-uniform)
+        uint32_t next_si = 0, next_rem = 0;
         c.st = ST_NEED;
         for (;;) {
             const uint64_t evm = ballot(c.st == ST_EVENT);
             const uint64_t trm = ballot(c.st == ST_TRACK);
-            const uint64_t ndm = next < total ? ballot(c.st == ST_NEED) : 0ull;
+            const uint64_t ndm = next_si < ns ? ballot(c.st == ST_NEED) : 0ull;
             if (!evm && !trm && !ndm) break;
             if ((evm || ndm) && ((uint32_t)__popcll(evm | ndm) >= a.event_threshold || !trm)) {
                 // handler pass: events, refill of finished lanes, ray set-ups
@@ -1118,15 +1127,21 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
                     if (c.st == ST_EVENT) handle_event<ADJ>(s, msrc, c, al, A, acc_sum, acc_sq, acc_grad, wd);
                 }
                 const uint64_t need = ballot(c.st == ST_NEED);
-                if (need && next < total) {
+                if (need && next_si < ns) {
                     if (c.st == ST_NEED) {
-                        uint32_t j = next + mbcnt(need);
-                        if (j < total)
-                            start_path<ADJ, FAST>(s, c, al, A, j, it);
+                        // triple next + mbcnt: the remainder stays below nslots + 64
+                        uint32_t r = next_rem + mbcnt(need);
+                        const uint32_t dq = udiv_small(r, it.r_nslots);
+                        r -= dq * it.nslots;
+                        if (next_si + dq < ns)
+                            start_path<ADJ, FAST>(s, c, al, A, next_si + dq, r, it);
                         else
                             c.st = ST_IDLE;
                     }
-                    next += (uint32_t)__popcll(need);
+                    const uint32_t adv = next_rem + (uint32_t)__popcll(need);
+                    const uint32_t dq = __builtin_amdgcn_readfirstlane(udiv_small(adv, it.r_nslots));
+                    next_si += dq;
+                    next_rem = adv - dq * it.nslots;
                 }
                 if (c.st == ST_RAY) ray_pass<FAST>(s, msrc, c);
             }
