@@ -982,10 +982,9 @@ __device__ __forceinline__ void start_path(const DevScene &s, LaneCtx &c, AdjLan
 // film rows of the item's pixels are contiguous ([pixel][band]), so the
 // stores coalesce; one lane per pixel sums its bands in band order.
 __device__ __forceinline__ void item_outputs(const DevScene &s, uint32_t p0, uint32_t np, uint32_t nslots,
-                                             const float *sum, const float *sq)
+                                             const float *sum, const float *sq, uint32_t lane)
 {
     const float inv = 1.0f / (float)s.spp;
-    const uint32_t lane = lane_id();
     const size_t base = (size_t)p0 * s.nlam;
     if (s.film)
         for (uint32_t i = lane; i < nslots; i += 64) s.film[base + i] = fmaxf(sum[i] * inv, 0.0f);
@@ -1038,7 +1037,6 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
     uint32_t loaded = 0xFFFFFFFFu;  // job whose majorants are in the wave's LDS copy
     // wave index, read into a scalar register: the wave's LDS bases are uniform
     const uint32_t wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t lane = lane_id();
     float *wave_lds = dyn_lds + wid * (kLdsFixed + a.maj_lds + 2 * a.shell_lds + a.lam_lds);
     float *acc_sum = wave_lds;
     float *acc_sq = wave_lds + kSlotsPerWave;
@@ -1053,6 +1051,12 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
     LaneCtx c;
     AdjLane al;
     for (;;) {
+        // the lane index, opaque to the compiler for each item: the per-lane
+        // LDS addresses of the item's set-up and output loops are then formed
+        // again per item instead of being held (or spilled) across the
+        // integration loop
+        uint32_t lane = lane_id();
+        asm volatile("" : "+v"(lane));
         uint32_t item = 0;
         if (lane == 0) item = atomicAdd(a.counter, 1u);
         item = __builtin_amdgcn_readfirstlane(item);
@@ -1161,7 +1165,7 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
             if (lane <= kNP) atomicAdd(&A.grad[lane], acc_grad[lane]);
         } else if (J.chunks == 1) {
             // the item holds every sample of its pixels: final values now
-            item_outputs(s, it.p0, np, it.nslots, acc_sum, acc_sq);
+            item_outputs(s, it.p0, np, it.nslots, acc_sum, acc_sq, lane);
         } else {
             float2 *dst = J.accum + ((size_t)ch * npx + it.p0) * s.nlam;
             for (uint32_t i = lane; i < it.nslots; i += 64) dst[i] = make_float2(acc_sum[i], acc_sq[i]);
