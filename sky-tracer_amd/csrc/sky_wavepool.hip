@@ -1035,8 +1035,9 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
     constexpr bool FAST = FK != 0;
     extern __shared__ float dyn_lds[];
     uint32_t loaded = 0xFFFFFFFFu;  // job whose majorants are in the wave's LDS copy
-    // wave index, read into a scalar register: the wave's LDS bases are uniform
-    const uint32_t wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // wave index, read into a scalar register so that the wave's LDS bases are
+    // scalars (the replay kernels, short of scalar registers, keep it per lane)
+    const uint32_t wid = ADJ ? threadIdx.x >> 6 : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     float *wave_lds = dyn_lds + wid * (kLdsFixed + a.maj_lds + 2 * a.shell_lds + a.lam_lds);
     float *acc_sum = wave_lds;
     float *acc_sq = wave_lds + kSlotsPerWave;
