@@ -41,7 +41,7 @@ PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
 # a 256 MiB copy): profiles/k_wavepool_c2_1024x512x64_counters.md.  Each work item writes its
 # pixels' XYZ (the 6.3 MB image, as partial cache lines shared with items on other XCDs) and
 # reads only the scene tables, so the figure does not depend on spp at the tuned work split.
-TRAFFIC_BYTES = {"c2": (2.0 * 1116.12 + 24256.0) * 1024}
+TRAFFIC_BYTES = {"c2": (2.0 * 101.44 + 16576.1) * 1024}
 # Same for the combo scan of all 6-band combos x 65536 samples:
 # profiles/k_scan_6band_combos_x_65536_samples_counters.md.
 SCAN_TRAFFIC_BYTES = {6: (2.0 * 5189454.0 + 70256.1) * 1024}
