@@ -193,3 +193,24 @@ def test_independent_seeds_give_disjoint_sample_keys():
             assert not keys[i] & keys[j]
     # seeds 1 and 2, by contrast, hand the same set of pixel keys to a 64-pixel image
     assert {p ^ 1 for p in range(64)} == {p ^ 2 for p in range(64)}
+
+
+def test_small_quotient_from_a_rounded_reciprocal():
+    """The integrator splits a triple index into (sample, pixel, band) and a
+    pixel into (x, y) with floor((n + 1/2) * rd), rd the correctly rounded
+    float 1 / d, in float32 arithmetic: (n + 1/2) / d lies at least 1/(2d)
+    from an integer and the product's error is below (n + 1/2) / d * 2^-22,
+    so the floor is exact for n < 2^21.  Checked exhaustively where the
+    kernel uses it (n < nslots + 64 <= 320, x < width + 256) and on random
+    n up to 2^21 for divisors up to 2^16."""
+    d = np.arange(1, 513, dtype=np.float32)
+    rd = (np.float32(1.0) / d).astype(np.float32)
+    n = np.arange(0, 1024, dtype=np.float32)
+    q = np.floor((n[None, :] + np.float32(0.5)) * rd[:, None]).astype(np.int64)
+    np.testing.assert_array_equal(q, np.arange(1024)[None, :] // np.arange(1, 513)[:, None])
+    rng = np.random.default_rng(3)
+    nn = rng.integers(0, 1 << 21, size=400000)
+    dd = rng.integers(1, 1 << 16, size=nn.size)
+    rdd = (np.float32(1.0) / dd.astype(np.float32)).astype(np.float32)
+    qq = np.floor((nn.astype(np.float32) + np.float32(0.5)) * rdd).astype(np.int64)
+    np.testing.assert_array_equal(qq, nn // dd)
