@@ -24,9 +24,11 @@
 //   NEED  : path finished, waits for the next triple;  IDLE : item drained.
 // The event handler runs when at least `event_threshold` lanes wait for it
 // (or nothing is tracking); up to `track_steps` tracking steps run between
-// two handler passes.  Per-(pixel, band) sums live in LDS (ds_add_f32) and
-// are written once per item to a [chunk][pixel][band] partial buffer that
-// k_finalize reduces in chunk order.
+// two handler passes.  Per-(pixel, band) sums live in LDS (ds_add_f32).  An
+// item that holds every sample of its pixels writes their film and XYZ itself
+// (one band per lane, XYZ by wave-level sums over each pixel's lanes); items
+// of a render split into sample chunks write a [chunk][pixel][band] partial
+// buffer that k_finalize reduces in chunk order.
 #include "sky_internal.hpp"
 #include "sky_trace_common.hpp"
 
@@ -976,11 +978,23 @@ __device__ __forceinline__ void start_path(const DevScene &s, LaneCtx &c, AdjLan
     }
 }
 
+// Sum of x over each aligned group of nl lanes (nl a power of two <= 64):
+// log2(nl) butterfly exchanges, after which every lane of a group holds the
+// group's sum.  All 64 lanes must take part.
+__device__ __forceinline__ float group_sum(float x, uint32_t nl)
+{
+    for (uint32_t o = nl >> 1; o > 0; o >>= 1) x += __shfl_xor(x, (int)o);
+    return x;
+}
+
 // End of a work item that holds every sample of its np pixels: the film
 // (1/spp, clamped at 0, integrator.rs:307-310), its second moment and
 // XYZ = sum_band film * cmf * weight, straight from the wave's LDS sums.  The
 // film rows of the item's pixels are contiguous ([pixel][band]), so the
-// stores coalesce; one lane per pixel sums its bands in band order.
+// stores coalesce.  XYZ: with a power-of-two band count of at least 4, one
+// band per lane and a wave-level sum over each pixel's lanes (64 slots at a
+// time); lane k then takes pixel k's sums and writes them.  Other band counts
+// sum a pixel's bands in band order on one lane.
 __device__ __forceinline__ void item_outputs(const DevScene &s, uint32_t p0, uint32_t np, uint32_t nslots,
                                              const float *sum, const float *sq, uint32_t lane)
 {
@@ -990,7 +1004,40 @@ __device__ __forceinline__ void item_outputs(const DevScene &s, uint32_t p0, uin
         for (uint32_t i = lane; i < nslots; i += 64) s.film[base + i] = fmaxf(sum[i] * inv, 0.0f);
     if (s.film_m2)
         for (uint32_t i = lane; i < nslots; i += 64) s.film_m2[base + i] = sq[i] * inv;
-    if (s.xyz) {
+    if (!s.xyz) return;
+    const uint32_t nl = s.nlam;
+    if (nl >= 4 && (nl & (nl - 1)) == 0) {
+        // nslots <= 256, so np <= 64: one lane per pixel at the end
+        const uint32_t ppc = 64u / nl;  // pixels per 64 slots
+        float X = 0.0f, Y = 0.0f, Z = 0.0f;
+        for (uint32_t c = 0; c * 64u < nslots; ++c) {
+            const uint32_t i = c * 64u + lane;
+            float x = 0.0f, y = 0.0f, z = 0.0f;
+            if (i < nslots) {
+                const float f = fmaxf(sum[i] * inv, 0.0f);
+                const DevLambda l = s.lam[lane & (nl - 1)];
+                x = f * l.wx;
+                y = f * l.wy;
+                z = f * l.wz;
+            }
+            x = group_sum(x, nl);
+            y = group_sum(y, nl);
+            z = group_sum(z, nl);
+            const int src = (int)((lane % ppc) * nl);  // first lane of this lane's pixel in chunk c
+            const float gx = __shfl(x, src), gy = __shfl(y, src), gz = __shfl(z, src);
+            if (lane / ppc == c) {
+                X = gx;
+                Y = gy;
+                Z = gz;
+            }
+        }
+        if (lane < np) {
+            float *o = s.xyz + 3 * (size_t)(p0 + lane);
+            o[0] = X;
+            o[1] = Y;
+            o[2] = Z;
+        }
+    } else {
         // (one 48-byte store per item instead of three strided ones measured
         // more HBM traffic: the item's bytes share cache lines with the
         // neighbouring items, which other XCDs write)
@@ -1249,6 +1296,7 @@ __global__ __launch_bounds__(256) void k_finalize(const DevJob *jobs)
     const uint32_t npix = min(P, npx - px0);
     const uint32_t t = threadIdx.x;
     const float inv = 1.0f / (float)s.spp;
+    float f = 0.0f;
     if (t < npix * nl) {
         const size_t e = (size_t)px0 * nl + t;
         float sum = 0.0f, sq = 0.0f;
@@ -1257,10 +1305,23 @@ __global__ __launch_bounds__(256) void k_finalize(const DevJob *jobs)
             sum += v.x;
             sq += v.y;
         }
-        const float f = fmaxf(sum * inv, 0.0f);
+        f = fmaxf(sum * inv, 0.0f);
         sf[t] = f;
         if (s.film) s.film[e] = f;
         if (s.film_m2) s.film_m2[e] = sq * inv;
+    }
+    if (s.xyz && nl >= 4 && (nl & (nl - 1)) == 0) {
+        // the same wave-level sums as the work items' outputs: entry t is
+        // band t % nl of pixel t / nl, groups of nl lanes lie inside a wave
+        const DevLambda l = s.lam[t & (nl - 1)];
+        const float X = group_sum(f * l.wx, nl), Y = group_sum(f * l.wy, nl), Z = group_sum(f * l.wz, nl);
+        if (t < npix * nl && (t & (nl - 1)) == 0) {
+            float *o = s.xyz + 3 * (size_t)(px0 + t / nl);
+            o[0] = X;
+            o[1] = Y;
+            o[2] = Z;
+        }
+        return;
     }
     __syncthreads();
     if (s.xyz && t < npix) {
