@@ -246,6 +246,7 @@ __device__ __forceinline__ ShellTab global_shells(const DevScene &s)
 struct Ray {
     double ox, oy, oz;  // origin (km), FP64
     double b, q;        // o.d and |o|^2 - (o.d)^2 - R^2
+    double inv;         // 1 / |d| (FP64) of the FP32 direction
     f3 d;               // unit direction
     float bf, qf, tmax;
     int kind;     // 0 ground, 1 atmosphere exit (sky_trace.wgsl:89-93)
@@ -265,6 +266,7 @@ __device__ __forceinline__ void ray_setup(const DevScene &s, const ShellTab &tab
 {
     double dx = d.x, dy = d.y, dz = d.z;
     double inv = inv_norm64(dx, dy, dz);
+    r.inv = inv;
     r.ox = ox;
     r.oy = oy;
     r.oz = oz;

@@ -191,10 +191,11 @@ __device__ __forceinline__ void add_dlog_phase(const DevScene &s, const AdjArgs 
 // removes a global-load latency from every shell crossing.
 constexpr uint32_t kMajLdsMax = 2048;
 // Per-wave LDS layout (floats): sums [256], sums of squares [256], adjoint
-// gradient [16 doubles], majorant grid [maj_lds], shell table [shell_lds
-// doubles].  The paths' gradient contributions cancel heavily across paths
-// (scores of both signs), so the item's sum is kept in FP64.
-constexpr uint32_t kLdsFixed = 2 * kSlotsPerWave + 32;
+// gradient [16 doubles], each lane's 1 / |d| of its current ray [64 doubles],
+// majorant grid [maj_lds], shell table [shell_lds doubles].  The paths'
+// gradient contributions cancel heavily across paths (scores of both signs),
+// so the item's sum is kept in FP64.
+constexpr uint32_t kLdsFixed = 2 * kSlotsPerWave + 32 + 128;
 
 typedef __attribute__((address_space(3))) const float *LdsFloatPtr;
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -214,6 +215,9 @@ struct MajSrc {
     // of being carried in registers by every lane
     LdsFloat4Ptr lam4;
     LdsFloatPtr lsun;
+    // FAST kernels: 1 / |d| (FP64) of each lane's current ray, kept from its
+    // set-up for the vertex update at the ray's end
+    __attribute__((address_space(3))) double *inv;
 };
 
 __device__ __forceinline__ float majorant(const MajSrc &m, uint32_t L, uint32_t band, int k)
@@ -246,6 +250,7 @@ __device__ __forceinline__ bool ray_begin(const DevScene &s, const MajSrc &m, La
 {
     Ray r;
     ray_setup<FAST ? (int)kFastMaxShells : 0>(s, m.shells, ox, oy, oz, d, r);
+    if constexpr (FAST) m.inv[lane_id()] = r.inv;
     c.d = d;
     c.bf = r.bf;
     c.qf = r.qf;
@@ -729,8 +734,9 @@ __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSr
     {
         // advance_vertex for a ground hit or an accepted collision (a lane
         // that does not move adds d * 0, which leaves its position exact)
+        // (1 / |d| as the ray's set-up computed it)
         const double dx = c.d.x, dy = c.d.y, dz = c.d.z;
-        const double f = inv_norm64(dx, dy, dz) * (double)(at_ground ? c.tmax : c.t);
+        const double f = m.inv[lane_id()] * (double)(at_ground ? c.tmax : c.t);
         const double fm = at_ground || hit ? f : 0.0;
         c.px += dx * fm;
         c.py += dy * fm;
@@ -1089,6 +1095,7 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
     float *acc_sum = wave_lds;
     float *acc_sq = wave_lds + kSlotsPerWave;
     double *acc_grad = (double *)(wave_lds + 2 * kSlotsPerWave);
+    double *lds_inv = (double *)(wave_lds + 2 * kSlotsPerWave + 32);
     float *lds_maj = wave_lds + kLdsFixed;
     double *lds_shell = (double *)(lds_maj + a.maj_lds);
     // FAST: [band] 4-float coefficients, then [band] sun radiance (kMaxLambda bands each)
@@ -1142,6 +1149,7 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
         msrc.shells.use_lds = msrc.use_lds;
         msrc.lam4 = (LdsFloat4Ptr)lds_lam4;
         msrc.lsun = (LdsFloatPtr)lds_lsun;
+        msrc.inv = (__attribute__((address_space(3))) double *)lds_inv;
         if (msrc.use_lds) {
             if (loaded != jid) {
                 for (uint32_t i = lane; i < nmaj; i += 64) lds_maj[i] = s.maj[i];
