@@ -239,7 +239,10 @@ __device__ __forceinline__ void seg_begin(const DevScene &s, const MajSrc &m, La
         tc = next_crossing(s, m.shells, c.q, c.bf, c.k, c.out, c.kn, c.on);
         c.maj = majorant(m, s.L, c.band, c.k);
     }
-    c.seg_end = fmaxf(fminf(tc, c.tmax), c.t);
+    // min(max(tc, t), tmax) as one median: every caller has t < tmax (a ray
+    // starts at 0 < tmax, a crossing is taken only below tmax), and tc is a
+    // finite root or +inf
+    c.seg_end = __builtin_amdgcn_fmed3f(tc, c.t, c.tmax);
 }
 
 // Sets up a ray from (ox, oy, oz) along d (FP64 set-up, sky_device.hpp) and
