@@ -355,9 +355,11 @@ __device__ __forceinline__ float next_crossing_sel(const DevScene &s, LdsDoubleP
     const double din = tab[k] - q;       // inner boundary of shell k
     const double dout = tab[k + 1] - q;  // outer boundary (the table has L + 1 entries)
     const bool inw = !out && din > 0.0;
-    const bool edge = inw ? k == 0 : k + 1 >= (int)s.L;
     on = inw ? 0 : 1;
-    kn = edge ? (inw ? 0 : k) : (inw ? k - 1 : k + 1);
+    // the next shell, clamped to [0, L - 1]: it stays k exactly at the two
+    // edges (inward from shell 0, outward from the top shell)
+    kn = min(max(inw ? k - 1 : k + 1, 0), (int)s.L - 1);
+    const bool edge = kn == k;
     const float r = f_sqrt((float)(inw ? din : (dout > 0.0 ? dout : 0.0)));
     const float tc = (inw ? -r : r) - bf;
     return edge ? __builtin_inff() : tc;
