@@ -296,7 +296,7 @@ __device__ __forceinline__ Coeffs lane_medium(const DevScene &s, const MajSrc &m
 // The collision is then evaluated and accepted or rejected in the same step.
 template <bool ADJ>
 __device__ __forceinline__ void track_step(const DevScene &s, const MajSrc &m, LaneCtx &c, AdjLane &a,
-                                           const AdjArgs &A, bool &wd)
+                                           const AdjArgs &A, uint32_t &wd)
 {
     float tn = 0.0f;
     const bool carried = (s.flags & kFlagCarriedTau) != 0;
@@ -319,7 +319,7 @@ __device__ __forceinline__ void track_step(const DevScene &s, const MajSrc &m, L
             return;
         }
         if (c.guard > s.max_segments) {
-            wd = true;
+            wd = 1u;
             c.st = ST_EVENT;
             c.aux = 0.0f;
             c.ev = c.mode == kTrackCollision ? EV_BOUNDARY : EV_SHADOW;
@@ -389,7 +389,7 @@ __device__ __forceinline__ void track_step(const DevScene &s, const MajSrc &m, L
 // form too.
 template <bool ADJ, bool CARRIED, bool RATIO>
 __device__ __forceinline__ void track_step_fast(const DevScene &s, const MajSrc &m, LaneCtx &c, AdjLane &a,
-                                                const AdjArgs &A, bool &wd)
+                                                const AdjArgs &A, uint32_t &wd)
 {
     float tn = 0.0f;
     if (c.tau < 0.0f && c.t < c.seg_end) c.tau = free_flight(rng_next(c.rng));
@@ -443,7 +443,7 @@ __device__ __forceinline__ void track_step_fast(const DevScene &s, const MajSrc 
     const bool pos = st > 0.0f && scat > 0.0f;
     const float tpw = pos ? c.tp * fminf(fmaxf(f_div(scat, st), 0.0f), 1.0f) : 0.0f;
     const bool dog = ended && c.t < c.tmax;  // watchdog: ends the path without contribution
-    wd = wd || dog;
+    wd |= dog ? 1u : 0u;
     c.t = col ? tn : c.t;
     c.tau = col ? -1.0f : c.tau;
     c.st = (ended || acc) ? ST_EVENT : c.st;
@@ -517,7 +517,7 @@ __device__ __forceinline__ void finish_path(LaneCtx &c, AdjLane &a, float *acc_s
 template <bool ADJ>
 __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m, LaneCtx &c, AdjLane &a,
                                              const AdjArgs &A, float *acc_sum, float *acc_sq, double *acc_grad,
-                                             bool &wd)
+                                             uint32_t &wd)
 {
     const uint32_t n_light = s.dls;
     const bool phase_is = (s.flags & kFlagPhaseIS) != 0;
@@ -543,7 +543,7 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
             c.nee_sum = 0.0f;
             nx = NX_NEE;
         } else if (c.kind == 3) {
-            wd = true;
+            wd = 1u;
         }
         // kind 2: tracking watchdog or a main ray without boundary, the path ends
     } else if (c.ev == EV_HIT) {
@@ -698,7 +698,7 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
         nx = NX_FINISH;
         if (!(s.max_depth && c.depth >= s.max_depth)) {
             if (c.depth >= s.watchdog_limit) {
-                wd = true;
+                wd = 1u;
             } else {
                 need_ray = true;
                 mode = kTrackCollision;
@@ -726,7 +726,7 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
 template <bool ADJ, bool IS, bool RATIO>
 __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSrc &m, LaneCtx &c, AdjLane &a,
                                                   const AdjArgs &A, float *acc_sum, float *acc_sq, double *acc_grad,
-                                                  bool &wd)
+                                                  uint32_t &wd)
 {
     const uint32_t n_light = s.dls;
     const float lsun = m.lsun[c.band];
@@ -759,7 +759,7 @@ __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSr
         a.J += (sun_seen ? a.w * Csun : 0.0f) + (gh ? a.w * Csh : 0.0f);
     }
     c.L = sun_seen ? c.L + c.tp * lsun : c.L;
-    wd = wd || (bnd && c.kind == 3);
+    wd |= bnd && c.kind == 3 ? 1u : 0u;
     const bool scatter_vertex = hit && c.tp > 0.0f;
     const bool nee_start = at_ground || scatter_vertex;
     c.vtype = at_ground ? 1 : (scatter_vertex ? 0 : c.vtype);
@@ -868,7 +868,7 @@ __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSr
     if (nx == NX_MAIN) {
         const bool capped = s.max_depth && c.depth >= s.max_depth;
         const bool dog = !capped && c.depth >= s.watchdog_limit;
-        wd = wd || dog;
+        wd |= dog ? 1u : 0u;
         main_ray = !capped && !dog;
         nx = main_ray ? NX_MAIN : NX_FINISH;
     }
@@ -1104,7 +1104,7 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
     // FAST: [band] 4-float coefficients, then [band] sun radiance (kMaxLambda bands each)
     v4f *lds_lam4 = (v4f *)(lds_maj + a.maj_lds + 2 * a.shell_lds);
     float *lds_lsun = (float *)(lds_lam4 + (uint32_t)kMaxLambda);
-    bool wd = false;
+    uint32_t wd = 0;  // a watchdog fired on this lane
     uint32_t *diag = nullptr;
     LaneCtx c;
     AdjLane al;
