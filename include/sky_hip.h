@@ -161,7 +161,8 @@ typedef struct sky_render_stats {
  * mean "use the default". */
 typedef struct sky_tuning {
     uint32_t waves_per_cu;    /* persistent waves per compute unit (24)        */
-    uint32_t pixels_per_item; /* pixels per wave work item (128 / count)       */
+    uint32_t pixels_per_item; /* pixels per wave work item (the fewest giving
+                                 16384 paths per item, at most 256 / count)   */
     uint32_t event_threshold; /* lanes waiting before events are handled (8)   */
     uint32_t track_steps;     /* tracking micro-steps between handler passes (2) */
     uint32_t occupancy;       /* forward renders: 0 the compiler's register budget; 4, 5, 6, 8 at

@@ -1376,11 +1376,22 @@ constexpr uint32_t kDefaultWavesPerCu = 24, kDefaultEventThreshold = 8, kDefault
 // kernel splits with float reciprocals: exact below 2^24.
 constexpr uint32_t kMaxItemTriples = 1u << 24;
 
+// Default pixels per work item: the fewest that give an item of all spp
+// samples at least 256 paths per lane (16384 triples).  Longer items spend
+// less of their time draining (the lanes idle while the item's last paths
+// finish); shorter ones leave less idle time at the end of the launch, when
+// the last items finish at different times.
+constexpr uint32_t kItemTriplesTarget = 16384;
+
 void plan_job(const DevScene &s, const sky_tuning &t, int num_cus, uint32_t min_items, DevJob &j)
 {
     const uint32_t npx = s.width * s.height;
     uint32_t cap = kSlotsPerWave / s.nlam;
-    uint32_t want = t.pixels_per_item ? t.pixels_per_item : (128u / s.nlam ? 128u / s.nlam : 1u);
+    const uint64_t per_pixel = (uint64_t)s.nlam * s.spp;
+    uint32_t want = t.pixels_per_item ? t.pixels_per_item
+                                      : (uint32_t)((kItemTriplesTarget + per_pixel - 1) / per_pixel > cap
+                                                       ? cap
+                                                       : (kItemTriplesTarget + per_pixel - 1) / per_pixel);
     j.pixels_per_item = want < 1 ? 1 : (want > cap ? cap : want);
     j.groups = (npx + j.pixels_per_item - 1) / j.pixels_per_item;
     // split samples only as far as needed for min_items, keeping >= 16 samples per item

@@ -32,7 +32,7 @@ SPACE = {
     "track_steps": [1, 2, 4, 8, 16, 32],
     "occupancy": [0, 4, 5, 6, 8],
     "waves_per_cu": [8, 12, 16, 24, 32],
-    "pixels_per_item": [0, 1, 2, 4, 8],
+    "pixels_per_item": [0, 1, 2, 4, 8],  # 0: the library's spp-aware default
 }
 
 
@@ -76,9 +76,13 @@ def _time(ctx: Context, scene: SceneData, config: RenderConfig, params: dict, re
 
 
 def autotune(scene: SceneData, config: RenderConfig, ctx: Context, probe_spp: int = 16,
-             rounds: int = 2, log=print, save: bool = True) -> Tuple[dict, float]:
+             rounds: int = 2, log=print, save: bool = True, split_spp: int = 0) -> Tuple[dict, float]:
     """Coordinate descent from the cached or default point; returns (params,
-    best kernel ms at probe_spp)."""
+    best kernel ms at probe_spp).  With split_spp, the work split (pixels
+    per item, 0 = the library's spp-aware default) is then chosen again at
+    that many samples per pixel: an item's length decides how much of it
+    drains with idle lanes and how long the launch's last items run, so the
+    split depends on the spp the renders actually use."""
     probe = dataclasses.replace(config, spp=probe_spp, sample_offset=0)
     cur = tuned(scene, config)
     best = _time(ctx, scene, probe, cur)
@@ -97,11 +101,26 @@ def autotune(scene: SceneData, config: RenderConfig, ctx: Context, probe_spp: in
                     best, cur, improved = ms, cand, True
         if not improved:
             break
+    split = None
+    if split_spp:
+        wide = dataclasses.replace(config, spp=split_spp, sample_offset=0)
+        times = {}
+        for v in SPACE["pixels_per_item"]:
+            times[v] = _time(ctx, scene, wide, dict(cur, pixels_per_item=v), reps=1)
+            log(f"[autotune] pixels_per_item={v} at {split_spp} spp: {times[v]:.1f} ms")
+        pick = min(times, key=times.get)
+        if 0 in times and times[0] <= times[pick] * 1.005:
+            pick = 0  # the spp-aware default, within 0.5% of the best here, adapts to other spp
+        if times[pick] < times[cur["pixels_per_item"]] * 0.995:
+            cur = dict(cur, pixels_per_item=pick)
+        split = {"spp": split_spp, "ms": {str(k): round(v, 1) for k, v in times.items()}}
     log(f"[autotune] best {cur} -> {best:.1f} ms")
     if save:
         cache = load_cache()
         cache[shape_class(scene, config)] = {"params": cur, "probe_ms": best, "probe_spp": probe_spp,
                                              "workload": f"{config.width}x{config.height}x{scene.band_count}"}
+        if split:
+            cache[shape_class(scene, config)]["split_probe"] = split
         with open(CACHE_PATH, "w") as f:
             json.dump(cache, f, indent=1, sort_keys=True)
     apply(ctx, config, cur)

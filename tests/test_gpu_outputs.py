@@ -33,7 +33,8 @@ def test_single_chunk_and_chunked_renders_agree(n_lambda, spp):
     whole.set_tuning(waves_per_cu=1, pixels_per_item=1)  # 1024 items wanted, one per pixel: one chunk
     a = _render(scene, cfg, chunked, want_m2=True)
     b = _render(scene, cfg, whole, want_m2=True)
-    ppi_a = min(max(128 // n_lambda, 1), 256 // n_lambda)
+    # the library's default split: the fewest pixels per item giving 16384 paths, at most 256 / bands
+    ppi_a = min(-(-16384 // (n_lambda * spp)), 256 // n_lambda)
     assert a.stats.items > (npx + ppi_a - 1) // ppi_a  # more items than pixel groups: chunked
     assert b.stats.items == npx  # one item per pixel: single chunk
     fa, fb = a.values().astype(np.float64), b.values().astype(np.float64)
