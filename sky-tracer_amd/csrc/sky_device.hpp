@@ -55,6 +55,7 @@ struct DevScene {
     float inv_2g, inv_dls;  // 1 / (2 g) (0 when |g| < 1e-3), 1 / direct_light_samples
     float l2_hr, l2_ha;   // log2(e) / H: exp(-h / H) = exp2(-h l2_h)
     float inv_w, inv_h;   // 1 / width, 1 / height
+    float inv_nlam;       // 1 / nlam
     // Sky-view LUT frame (sky_trace.wgsl:641-681), precomputed on the host.
     float svl_zha, svl_beta, svl_fw[3], svl_side[3];
     uint32_t width, height, spp, sample_offset;
@@ -523,10 +524,14 @@ __device__ __forceinline__ f3 camera_dir(const DevScene &s, uint32_t x, uint32_t
                                          float j2)
 {
     if (s.projection == 1u) {
-        float u = ((float)x + 0.5f) * s.inv_w;
-        float v = ((float)y + 0.5f) * s.inv_h;
-        float uu = s.width <= 1 ? 0.5f : f_div(u - 0.5f * s.inv_w, 1.0f - s.inv_w);
-        float vv = s.height <= 1 ? 0.5f : f_div(v - 0.5f * s.inv_h, 1.0f - s.inv_h);
+        // opaque copies: the terms below are then formed here, not hoisted
+        // out of the caller's loops into registers held across them
+        float iw = s.inv_w, ih = s.inv_h;
+        asm volatile("" : "+v"(iw), "+v"(ih));
+        float u = ((float)x + 0.5f) * iw;
+        float v = ((float)y + 0.5f) * ih;
+        float uu = s.width <= 1 ? 0.5f : f_div(u - 0.5f * iw, 1.0f - iw);
+        float vv = s.height <= 1 ? 0.5f : f_div(v - 0.5f * ih, 1.0f - ih);
         uu = fminf(fmaxf(uu, 0.0f), 1.0f);
         vv = fminf(fmaxf(vv, 0.0f), 1.0f);
         float vzc;

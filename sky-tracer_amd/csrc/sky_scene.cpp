@@ -590,6 +590,7 @@ sky_status build_scene(const sky_planet *planet, const sky_atmosphere *a, const 
     s.seed_lo = (uint32_t)cfg->seed;
     s.seed_hi = (uint32_t)(cfg->seed >> 32);
     s.nlam = wl->count;
+    s.inv_nlam = (float)(1.0 / wl->count);
     s.dls = cfg->direct_light_samples ? cfg->direct_light_samples : 1u;  // integrator.rs:325
     s.inv_dls = 1.0f / (float)s.dls;
     s.projection = cfg->projection;

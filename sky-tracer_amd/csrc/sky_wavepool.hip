@@ -192,14 +192,16 @@ __device__ __forceinline__ void add_dlog_phase(const DevScene &s, const AdjArgs 
 constexpr uint32_t kMajLdsMax = 2048;
 // Per-wave LDS layout (floats): sums [256], sums of squares [256], adjoint
 // gradient [16 doubles], each lane's 1 / |d| of its current ray [64 doubles],
-// majorant grid [maj_lds], shell table [shell_lds doubles].  The paths'
-// gradient contributions cancel heavily across paths (scores of both signs),
-// so the item's sum is kept in FP64.
-constexpr uint32_t kLdsFixed = 2 * kSlotsPerWave + 32 + 128;
+// the ring of path starts [64 x 4], majorant grid [maj_lds], shell table
+// [shell_lds doubles].  The paths' gradient contributions cancel heavily
+// across paths (scores of both signs), so the item's sum is kept in FP64.
+constexpr uint32_t kRingSlots = 64;
+constexpr uint32_t kLdsFixed = 2 * kSlotsPerWave + 32 + 128 + 4 * kRingSlots;
 
 typedef __attribute__((address_space(3))) const float *LdsFloatPtr;
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) const v4f *LdsFloat4Ptr;
+typedef __attribute__((address_space(3))) v4f *LdsRingPtr;
 typedef __attribute__((address_space(1))) const float *GlobalFloatPtr;
 
 // The two sources carry their address spaces so that each load is a plain
@@ -920,16 +922,18 @@ struct ItemInfo {
     float r_nslots, r_nlam, r_width;
 };
 
-// Path start (main, sky_trace.wgsl:799-807): triple j = si * nslots + rem of
-// the item, ordered (sample, pixel, band) with the band fastest.
-template <bool ADJ, bool FAST>
-__device__ __forceinline__ void start_path(const DevScene &s, LaneCtx &c, AdjLane &a, const AdjArgs &A,
-                                           uint32_t si, uint32_t rem, const ItemInfo &it)
+// Path start (main, sky_trace.wgsl:799-807) in two parts.  A start is triple
+// j = si * nslots + rem of the item, ordered (sample, pixel, band) with the
+// band fastest.  make_start forms its camera ray -- the two jitter draws and
+// the direction -- ahead of need: the whole wave makes 64 consecutive starts
+// at once into the wave's ring, {dir.x, dir.y, dir.z, rem | si << 8}
+// (rem < nslots <= 256, si < 2^24).  start_path gives a lane whose path
+// ended the next start of the ring.  Lanes take starts in triple order either
+// way, so the ring changes no path and no summation order.
+__device__ __forceinline__ v4f make_start(const DevScene &s, uint32_t si, uint32_t rem, const ItemInfo &it)
 {
-    // rem < nslots <= 256
-    uint32_t pl = udiv_small(rem, it.r_nlam);
-    c.band = rem - pl * s.nlam;
-    c.slot = rem;
+    const uint32_t pl = udiv_small(rem, it.r_nlam);
+    const uint32_t band = rem - pl * s.nlam;
     // pixel p0 + pl, from the item's first pixel (x0, y0); pl < 256
     uint32_t x = it.x0 + pl, y = it.y0;
     if (s.width >= 256u) {
@@ -942,6 +946,22 @@ __device__ __forceinline__ void start_path(const DevScene &s, LaneCtx &c, AdjLan
         x -= dy * s.width;
         y += dy;
     }
+    const Rng r = rng_make(it.p0 + pl, it.s0 + si, band, s);
+    const float j1 = rng_peek(r, 0);
+    const float j2 = rng_peek(r, 1);
+    const f3 d = camera_dir(s, x, y, j1, j2);
+    return v4f{d.x, d.y, d.z, __uint_as_float(rem | (si << 8))};
+}
+
+template <bool ADJ, bool FAST>
+__device__ __forceinline__ void start_path(const DevScene &s, LaneCtx &c, AdjLane &a, const AdjArgs &A,
+                                           const v4f e, const ItemInfo &it)
+{
+    const uint32_t key = __float_as_uint(e.w);
+    const uint32_t rem = key & 255u, si = key >> 8;
+    const uint32_t pl = udiv_small(rem, it.r_nlam);
+    c.band = rem - pl * s.nlam;
+    c.slot = rem;
     const uint32_t pixel = it.p0 + pl;
     if constexpr (!FAST) {
         const DevLambda l = s.lam[c.band];
@@ -951,11 +971,10 @@ __device__ __forceinline__ void start_path(const DevScene &s, LaneCtx &c, AdjLan
         c.baa = l.baa;
         c.lsun = l.lsun;
     }
+    // the key words of rng_make, past the camera's two jitter draws
     c.rng = rng_make(pixel, it.s0 + si, c.band, s);
-    float j1 = rng_peek(c.rng, 0);
-    float j2 = rng_peek(c.rng, 1);
     c.rng.w += 2u * kLcgMul;
-    c.vin = camera_dir(s, x, y, j1, j2);
+    c.vin = f3{e.x, e.y, e.z};
     c.px = 0.0;
     c.py = s.obs_y;
     c.pz = 0.0;
@@ -1099,6 +1118,7 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
     float *acc_sq = wave_lds + kSlotsPerWave;
     double *acc_grad = (double *)(wave_lds + 2 * kSlotsPerWave);
     double *lds_inv = (double *)(wave_lds + 2 * kSlotsPerWave + 32);
+    const LdsRingPtr ring = (LdsRingPtr)(wave_lds + 2 * kSlotsPerWave + 32 + 128);
     float *lds_maj = wave_lds + kLdsFixed;
     double *lds_shell = (double *)(lds_maj + a.maj_lds);
     // FAST: [band] 4-float coefficients, then [band] sun radiance (kMaxLambda bands each)
@@ -1135,9 +1155,11 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
         it.s0 = s.sample_offset + ch * J.samples_per_chunk;
         const uint32_t ns = min(J.samples_per_chunk, s.spp - ch * J.samples_per_chunk);
         it.nslots = np * s.nlam;
-        it.r_nslots = 1.0f / (float)it.nslots;
-        it.r_nlam = 1.0f / (float)s.nlam;
-        it.r_width = 1.0f / (float)s.width;
+        // reciprocals for udiv_small (within an ulp): the hardware reciprocal,
+        // kept in a scalar register, and the scene's host-side ones
+        it.r_nslots = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(f_rcp((float)it.nslots))));
+        it.r_nlam = s.inv_nlam;
+        it.r_width = s.inv_w;
         for (uint32_t i = lane; i < it.nslots; i += 64) {
             acc_sum[i] = 0.0f;
             acc_sq[i] = 0.0f;
@@ -1172,13 +1194,16 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        // next triple to hand out, next_si * nslots + next_rem (wave-uniform)
-        uint32_t next_si = 0, next_rem = 0;
+        // Wave-uniform: the next triple to make a start for, fill_si * nslots +
+        // fill_rem, and the ring's unused starts, `avail` from slot `head` on
+        // (slots mod kRingSlots).
+        uint32_t fill_si = 0, fill_rem = 0, head = 0, avail = 0;
         c.st = ST_NEED;
         for (;;) {
+            const bool more = avail != 0 || fill_si < ns;
             const uint64_t evm = ballot(c.st == ST_EVENT);
             const uint64_t trm = ballot(c.st == ST_TRACK);
-            const uint64_t ndm = next_si < ns ? ballot(c.st == ST_NEED) : 0ull;
+            const uint64_t ndm = more ? ballot(c.st == ST_NEED) : 0ull;
             if (!evm && !trm && !ndm) break;
             if ((evm || ndm) && ((uint32_t)__popcll(evm | ndm) >= a.event_threshold || !trm)) {
                 // handler pass: events, refill of finished lanes, ray set-ups
@@ -1190,21 +1215,49 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
                     if (c.st == ST_EVENT) handle_event<ADJ>(s, msrc, c, al, A, acc_sum, acc_sq, acc_grad, wd);
                 }
                 const uint64_t need = ballot(c.st == ST_NEED);
-                if (need && next_si < ns) {
+                if (need && more) {
+                    // the k-th lane in NEED takes ring slot head + k: first the
+                    // unused starts, then, if they do not cover every lane, the
+                    // next 64 starts, made by the whole wave into the slots after
+                    // them (the slots just read are free again)
+                    const uint32_t n_need = (uint32_t)__popcll(need);
                     if (c.st == ST_NEED) {
-                        // triple next + mbcnt: the remainder stays below nslots + 64
-                        uint32_t r = next_rem + mbcnt(need);
-                        const uint32_t dq = udiv_small(r, it.r_nslots);
-                        r -= dq * it.nslots;
-                        if (next_si + dq < ns)
-                            start_path<ADJ, FAST>(s, c, al, A, next_si + dq, r, it);
-                        else
-                            c.st = ST_IDLE;
+                        const uint32_t rank = mbcnt(need);
+                        if (rank < avail) start_path<ADJ, FAST>(s, c, al, A, ring[(head + rank) & (kRingSlots - 1)], it);
                     }
-                    const uint32_t adv = next_rem + (uint32_t)__popcll(need);
-                    const uint32_t dq = __builtin_amdgcn_readfirstlane(udiv_small(adv, it.r_nslots));
-                    next_si += dq;
-                    next_rem = adv - dq * it.nslots;
+                    if (n_need > avail) {
+                        const uint32_t base = head + avail;
+                        uint32_t have = 0;  // starts made (triples past the item's end are not)
+                        if (fill_si < ns) {
+                            const uint32_t r = fill_rem + lane;  // < nslots + 64
+                            const uint32_t dq = udiv_small(r, it.r_nslots);
+                            const uint32_t si = fill_si + dq;
+                            ring[(base + lane) & (kRingSlots - 1)] = make_start(s, si, r - dq * it.nslots, it);
+                            have = (uint32_t)__popcll(ballot(si < ns));
+                            const uint32_t adv = fill_rem + kRingSlots;
+                            const uint32_t dqa = __builtin_amdgcn_readfirstlane(udiv_small(adv, it.r_nslots));
+                            fill_si += dqa;
+                            fill_rem = adv - dqa * it.nslots;
+                            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                            __builtin_amdgcn_wave_barrier();
+                            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                        }
+                        // the lanes still in NEED, in the same order
+                        const uint64_t rest = ballot(c.st == ST_NEED);
+                        if (c.st == ST_NEED) {
+                            const uint32_t rank = mbcnt(rest);
+                            if (rank < have)
+                                start_path<ADJ, FAST>(s, c, al, A, ring[(base + rank) & (kRingSlots - 1)], it);
+                            else
+                                c.st = ST_IDLE;
+                        }
+                        const uint32_t used = n_need - avail < have ? n_need - avail : have;
+                        head = base + used;
+                        avail = have - used;
+                    } else {
+                        head += n_need;
+                        avail -= n_need;
+                    }
                 }
                 if (c.st == ST_RAY) ray_pass<FAST>(s, msrc, c);
             }
