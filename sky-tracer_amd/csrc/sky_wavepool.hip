@@ -441,25 +441,27 @@ __device__ __forceinline__ void track_step_fast(const DevScene &s, const MajSrc 
     const float u = rng_peek(c.rng, 0);
     c.rng.w += col && !rt ? kLcgMul : 0u;
     const bool acc = col && !rt && u < ratio;
-    const float scat = co.sr + co.sas;
-    const bool pos = st > 0.0f && scat > 0.0f;
-    const float tpw = pos ? c.tp * fminf(fmaxf(f_div(scat, st), 0.0f), 1.0f) : 0.0f;
     const bool dog = ended && c.t < c.tmax;  // watchdog: ends the path without contribution
     wd |= dog ? 1u : 0u;
     c.t = col ? tn : c.t;
     c.tau = col ? -1.0f : c.tau;
     c.st = (ended || acc) ? ST_EVENT : c.st;
     c.ev = ended ? (main ? EV_BOUNDARY : EV_SHADOW) : (acc ? (main ? EV_HIT : EV_SHADOW) : c.ev);
-    c.tp = acc && main ? tpw : c.tp;
-    c.rw = acc && main && pos ? f_div(co.sr, scat) : c.rw;
-    c.aux = dog || (acc && !main) ? 0.0f : (rt && col ? c.aux * (1.0f - ratio) : c.aux);
+    // an accepted main-ray collision keeps sigma_R, sigma_As and sigma_t in
+    // rw, nee_factor and aux (free during a main ray): the handler forms the
+    // vertex's albedo weight and Rayleigh share from them
+    const bool vert = acc && main;
+    c.rw = vert ? co.sr : c.rw;
+    c.nee_factor = vert ? co.sas : c.nee_factor;
+    c.aux = vert ? st : (dog || (acc && !main) ? 0.0f : (rt && col ? c.aux * (1.0f - ratio) : c.aux));
     c.kind = dog ? 2 : c.kind;
     if constexpr (ADJ) {
         // accepted main-ray collision: d log(sigma_s / mu) into S; null
         // collision: d log(1 - sigma_t / mu) into S (main ray) or SS (shadow)
         const DSig d = dsig(s, A, c.band, h, m.lam4[c.band].y, co);
         const bool rej = col && !acc;
-        const float fs = acc && main && scat > 0.0f ? 1.0f / scat : 0.0f;
+        const float scat = co.sr + co.sas;
+        const float fs = vert && scat > 0.0f ? 1.0f / scat : 0.0f;
         const float fr = rej ? -1.0f / fmaxf(c.maj - st, 1e-30f) : 0.0f;
         const float fm = main ? fr : 0.0f, fh = main ? 0.0f : fr;
         a.S[0] += (fs + fm) * d.r0;
@@ -471,7 +473,6 @@ __device__ __forceinline__ void track_step_fast(const DevScene &s, const MajSrc 
         a.S[6] += fs * d.as6 + fm * (d.as6 + d.aa6);
         a.S[8] += fs * d.as8 + fm * (d.as8 + d.aa8);
         add_dsigma_t(a.SS, d, fh);
-        const bool vert = acc && main;
         a.vsr = vert ? co.sr : a.vsr;
         a.vsas = vert ? co.sas : a.vsas;
         a.vh = vert ? h : a.vh;
@@ -736,6 +737,15 @@ __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSr
     // 1. the event
     const bool bnd = c.ev == EV_BOUNDARY, hit = c.ev == EV_HIT, shd = !bnd && !hit;
     const bool at_ground = bnd && c.kind == 0;
+    {
+        // an accepted collision: the albedo weight and the vertex's Rayleigh
+        // share (wgsl:722-731) from sigma_R, sigma_As and sigma_t kept by the
+        // tracking step; a zero throughput ends the path below
+        const float sr = c.rw, st = c.aux, scat = sr + c.nee_factor;
+        const bool pos = st > 0.0f && scat > 0.0f;
+        c.tp = hit ? (pos ? c.tp * fminf(fmaxf(f_div(scat, st), 0.0f), 1.0f) : 0.0f) : c.tp;
+        c.rw = hit && pos ? f_div(sr, scat) : c.rw;
+    }
     {
         // advance_vertex for a ground hit or an accepted collision (a lane
         // that does not move adds d * 0, which leaves its position exact)
