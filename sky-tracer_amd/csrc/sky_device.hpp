@@ -353,15 +353,19 @@ __device__ __forceinline__ float next_crossing(const DevScene &s, const ShellTab
 __device__ __forceinline__ float next_crossing_sel(const DevScene &s, LdsDoublePtr tab, double q, float bf, int k,
                                                    int out, int &kn, int &on)
 {
-    const double din = tab[k] - q;       // inner boundary of shell k
-    const double dout = tab[k + 1] - q;  // outer boundary (the table has L + 1 entries)
-    const bool inw = !out && din > 0.0;
+    const double din = tab[k];       // inner boundary of shell k
+    const double dout = tab[k + 1];  // outer boundary (the table has L + 1 entries)
+    // inward iff the ray reaches the inner boundary: din - q > 0, which for
+    // finite operands holds exactly when din > q; then one subtraction for
+    // the chosen boundary (positive when inward)
+    const bool inw = !out && din > q;
     on = inw ? 0 : 1;
     // the next shell, clamped to [0, L - 1]: it stays k exactly at the two
     // edges (inward from shell 0, outward from the top shell)
     kn = min(max(inw ? k - 1 : k + 1, 0), (int)s.L - 1);
     const bool edge = kn == k;
-    const float r = f_sqrt((float)(inw ? din : (dout > 0.0 ? dout : 0.0)));
+    const double d = (inw ? din : dout) - q;
+    const float r = f_sqrt((float)(d > 0.0 ? d : 0.0));
     const float tc = (inw ? -r : r) - bf;
     return edge ? __builtin_inff() : tc;
 }

@@ -235,8 +235,10 @@ __device__ __forceinline__ void seg_begin(const DevScene &s, const MajSrc &m, La
 {
     float tc;
     if constexpr (FAST) {
-        tc = next_crossing_sel(s, m.shells.lds, c.q, c.bf, c.k, c.out, c.kn, c.on);
+        // the shell's majorant first: its LDS read overlaps the crossing's
+        // FP64 arithmetic
         c.maj = m.lds[c.band * s.L + (uint32_t)c.k];
+        tc = next_crossing_sel(s, m.shells.lds, c.q, c.bf, c.k, c.out, c.kn, c.on);
     } else {
         tc = next_crossing(s, m.shells, c.q, c.bf, c.k, c.out, c.kn, c.on);
         c.maj = majorant(m, s.L, c.band, c.k);
