@@ -81,6 +81,7 @@ struct LaneCtx {
     float bf, qf, tmax;
     double q;
     float t, tau, seg_end, maj;
+    float rm;  // FAST kernels: 1 / maj, from the LDS majorant table
     float aux;  // shadow-ray transmittance (0/1 delta, product for ratio tracking)
     int k, out, kn, on;
     uint32_t guard;  // shell-crossing watchdog (see ray_begin)
@@ -192,8 +193,8 @@ __device__ __forceinline__ void add_dlog_phase(const DevScene &s, const AdjArgs 
 constexpr uint32_t kMajLdsMax = 2048;
 // Per-wave LDS layout (floats): sums [256], sums of squares [256], adjoint
 // gradient [16 doubles], each lane's 1 / |d| of its current ray [64 doubles],
-// the ring of path starts [64 x 4], majorant grid [maj_lds], shell table
-// [shell_lds doubles].  The paths' gradient contributions cancel heavily
+// the ring of path starts [64 x 4], majorant grid [maj_lds] (in the FAST
+// kernels as (majorant, reciprocal) pairs), shell table [shell_lds doubles].  The paths' gradient contributions cancel heavily
 // across paths (scores of both signs), so the item's sum is kept in FP64.
 constexpr uint32_t kRingSlots = 64;
 constexpr uint32_t kLdsFixed = 2 * kSlotsPerWave + 32 + 128 + 4 * kRingSlots;
@@ -202,6 +203,8 @@ typedef __attribute__((address_space(3))) const float *LdsFloatPtr;
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) const v4f *LdsFloat4Ptr;
 typedef __attribute__((address_space(3))) v4f *LdsRingPtr;
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) const v2f *LdsFloat2Ptr;
 typedef __attribute__((address_space(1))) const float *GlobalFloatPtr;
 
 // The two sources carry their address spaces so that each load is a plain
@@ -209,6 +212,9 @@ typedef __attribute__((address_space(1))) const float *GlobalFloatPtr;
 // which waits on both memory counters).
 struct MajSrc {
     LdsFloatPtr lds;      // wave's LDS copy (when use_lds)
+    // FAST kernels: the copy as (majorant, its hardware reciprocal) pairs, so
+    // that a segment's free-flight arithmetic takes no reciprocal per trip
+    LdsFloat2Ptr lds2;
     GlobalFloatPtr glob;  // the job's grid in global memory
     bool use_lds;
     ShellTab shells;      // the job's shell table, LDS copy with the grid
@@ -229,14 +235,22 @@ __device__ __forceinline__ float majorant(const MajSrc &m, uint32_t L, uint32_t 
     return m.glob[i];
 }
 
-// FAST: the launch holds every job's tables in LDS (no global fallback).
-template <bool FAST>
+// MJ: 0 the general kernels; FAST kernels (every job's tables in LDS, no
+// global fallback) with the majorant table as single floats (1: the
+// ratio-tracked kernels, whose larger state leaves no register for 1 / maj)
+// or as (majorant, reciprocal) pairs (2).
+template <int MJ>
 __device__ __forceinline__ void seg_begin(const DevScene &s, const MajSrc &m, LaneCtx &c)
 {
     float tc;
-    if constexpr (FAST) {
-        // the shell's majorant first: its LDS read overlaps the crossing's
-        // FP64 arithmetic
+    if constexpr (MJ == 2) {
+        // the shell's majorant and its reciprocal first: the LDS read overlaps
+        // the crossing's FP64 arithmetic
+        const v2f mm = m.lds2[c.band * s.L + (uint32_t)c.k];
+        c.maj = mm.x;
+        c.rm = mm.y;
+        tc = next_crossing_sel(s, m.shells.lds, c.q, c.bf, c.k, c.out, c.kn, c.on);
+    } else if constexpr (MJ == 1) {
         c.maj = m.lds[c.band * s.L + (uint32_t)c.k];
         tc = next_crossing_sel(s, m.shells.lds, c.q, c.bf, c.k, c.out, c.kn, c.on);
     } else {
@@ -251,13 +265,13 @@ __device__ __forceinline__ void seg_begin(const DevScene &s, const MajSrc &m, La
 
 // Sets up a ray from (ox, oy, oz) along d (FP64 set-up, sky_device.hpp) and
 // starts tracking it; returns false if it has no boundary.
-template <bool FAST>
+template <int MJ>
 __device__ __forceinline__ bool ray_begin(const DevScene &s, const MajSrc &m, LaneCtx &c, double ox,
                                           double oy, double oz, f3 d, int mode)
 {
     Ray r;
-    ray_setup<FAST ? (int)kFastMaxShells : 0>(s, m.shells, ox, oy, oz, d, r);
-    if constexpr (FAST) m.inv[lane_id()] = r.inv;
+    ray_setup<MJ ? (int)kFastMaxShells : 0>(s, m.shells, ox, oy, oz, d, r);
+    if constexpr (MJ != 0) m.inv[lane_id()] = r.inv;
     c.d = d;
     c.bf = r.bf;
     c.qf = r.qf;
@@ -269,10 +283,10 @@ __device__ __forceinline__ bool ray_begin(const DevScene &s, const MajSrc &m, La
     c.tau = -1.0f;
     c.aux = 1.0f;
     // crossings made so far (from 1), or in the FAST walk the crossings left
-    c.guard = FAST ? s.max_segments : 1u;
+    c.guard = MJ ? s.max_segments : 1u;
     c.k = r.layer;
     c.out = r.outward;
-    seg_begin<FAST>(s, m, c);
+    seg_begin<MJ>(s, m, c);
     c.st = ST_TRACK;
     return r.hit != 0;
 }
@@ -333,7 +347,7 @@ __device__ __forceinline__ void track_step(const DevScene &s, const MajSrc &m, L
         c.guard++;
         c.k = c.kn;
         c.out = c.on;
-        seg_begin<false>(s, m, c);
+        seg_begin<0>(s, m, c);
     }
     c.t = tn;
     const float h = lane_altitude(s, c, tn);
@@ -398,12 +412,13 @@ __device__ __forceinline__ void track_step_fast(const DevScene &s, const MajSrc 
     float tn = 0.0f;
     if (c.tau < 0.0f && c.t < c.seg_end) c.tau = free_flight(rng_next(c.rng));
     bool ended = false;
-    // 1 / majorant of the current segment; the loop leaves with the segment
-    // of the collision or of the ray's end, whose ratio below reuses it
+    // 1 / majorant of the current segment (the table's f_rcp(maj)); the loop
+    // leaves with the segment of the collision or of the ray's end, whose
+    // ratio below reuses it
     float rm;
     for (;;) {
         const bool nonempty = c.t < c.seg_end;
-        rm = f_rcp(c.maj);
+        rm = RATIO ? f_rcp(c.maj) : c.rm;
         tn = c.t + c.tau * rm;
         if constexpr (CARRIED) {
             const float seg_tau = c.maj * (c.seg_end - c.t);
@@ -424,7 +439,7 @@ __device__ __forceinline__ void track_step_fast(const DevScene &s, const MajSrc 
         c.guard--;
         c.k = c.kn;
         c.out = c.on;
-        seg_begin<true>(s, m, c);
+        seg_begin<RATIO ? 1 : 2>(s, m, c);
         if (c.tau < 0.0f && c.t < c.seg_end) c.tau = free_flight(rng_next(c.rng));
     }
     // The end of the ray, or the tentative collision and its accept test
@@ -907,13 +922,13 @@ __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSr
 // main ray from the current origin (vertex already moved, or the camera).
 // A shadow ray that does not leave the atmosphere is a zero sun sample; a
 // main ray with no boundary ends the path at the next pass.
-template <bool FAST>
+template <int MJ>
 __device__ __forceinline__ void ray_pass(const DevScene &s, const MajSrc &m, LaneCtx &c)
 {
     const bool shadow = c.mode != kTrackCollision;
     const f3 dir = c.d;
     const double e = shadow ? (double)kRayEpsilonKm : 0.0;
-    bool hit = ray_begin<FAST>(s, m, c, c.px + (double)dir.x * e, c.py + (double)dir.y * e, c.pz + (double)dir.z * e, dir,
+    bool hit = ray_begin<MJ>(s, m, c, c.px + (double)dir.x * e, c.py + (double)dir.y * e, c.pz + (double)dir.z * e, dir,
                          c.mode);
     if (shadow) {
         if (!(hit && c.kind == 1)) {
@@ -1120,6 +1135,7 @@ template <bool ADJ, int FK>
 __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &A)
 {
     constexpr bool FAST = FK != 0;
+    constexpr int MJ = FK == 0 ? 0 : (FK == 3 ? 1 : 2);  // majorant table form (seg_begin)
     extern __shared__ float dyn_lds[];
     uint32_t loaded = 0xFFFFFFFFu;  // job whose majorants are in the wave's LDS copy
     // wave index, read into a scalar register so that the wave's LDS bases are
@@ -1178,6 +1194,7 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
         }
         MajSrc msrc;
         msrc.lds = (LdsFloatPtr)lds_maj;
+        msrc.lds2 = (LdsFloat2Ptr)lds_maj;
         msrc.glob = (GlobalFloatPtr)s.maj;
         const uint32_t nmaj = s.nlam * s.L;
         msrc.use_lds = FAST || (nmaj <= a.maj_lds && s.L + 1 <= a.shell_lds);
@@ -1189,7 +1206,14 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
         msrc.inv = (__attribute__((address_space(3))) double *)lds_inv;
         if (msrc.use_lds) {
             if (loaded != jid) {
-                for (uint32_t i = lane; i < nmaj; i += 64) lds_maj[i] = s.maj[i];
+                if constexpr (MJ == 2) {
+                    for (uint32_t i = lane; i < nmaj; i += 64) {
+                        const float mj = s.maj[i];
+                        ((v2f *)lds_maj)[i] = v2f{mj, f_rcp(mj)};
+                    }
+                } else {
+                    for (uint32_t i = lane; i < nmaj; i += 64) lds_maj[i] = s.maj[i];
+                }
                 for (uint32_t i = lane; i <= s.L; i += 64) lds_shell[i] = s.shell_d[i];
                 if constexpr (FAST) {
                     for (uint32_t i = lane; i < s.nlam; i += 64) {
@@ -1271,7 +1295,7 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
                         avail -= n_need;
                     }
                 }
-                if (c.st == ST_RAY) ray_pass<FAST>(s, msrc, c);
+                if (c.st == ST_RAY) ray_pass<MJ>(s, msrc, c);
             }
             for (uint32_t i = 0; i < a.track_steps; ++i) {
                 if (!ballot(c.st == ST_TRACK)) break;
@@ -1495,6 +1519,7 @@ WavePlan plan_launch(const sky_tuning &t, int num_cus, uint32_t total_items, uin
              : est == (kFlagCarriedTau | kFlagPhaseIS | kFlagRatio) ? 3
                                                                       : 0;
     p.shell_lds = p.maj_lds ? ((max_layers + 1u + 7u) & ~7u) : 0u;
+    if (p.fast == 1 || p.fast == 2) p.maj_lds *= 2;  // (majorant, reciprocal) pairs
     p.lam_lds = p.fast ? 5u * (uint32_t)kMaxLambda : 0u;
     p.lds_bytes = kWavesPerBlock * (kLdsFixed + p.maj_lds + 2 * p.shell_lds + p.lam_lds) * (uint32_t)sizeof(float);
     uint32_t wpc = t.waves_per_cu ? t.waves_per_cu : kDefaultWavesPerCu;
