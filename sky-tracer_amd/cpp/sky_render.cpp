@@ -127,7 +127,7 @@ int main(int argc, char **argv)
         // majorant shells to the tuned ones.
         sky_tuning t{};
         t.waves_per_cu = tabulated ? 32 : 24;
-        t.pixels_per_item = tabulated ? 8 : 4;
+        t.pixels_per_item = tabulated ? 8 : 0;  // 0: the library's spp-aware work split
         t.event_threshold = tabulated ? 24 : 8;
         t.track_steps = 2;
         t.occupancy = 6;
