@@ -4,9 +4,11 @@
 //
 // Precision split: ray set-up (origin, b = o.d, q = |o|^2 - b^2 - R^2, boundary
 // distances, shell discriminants) is FP64; every per-step quantity is FP32.
-// The altitude along a ray is r^2 - R^2 = q + (b + t)^2, which has no
-// cancellation at 6360 km (the reference's (r-R)(r+R) at sky_trace.wgsl:214 is
-// FP32 and loses ~0.5 m per operand there).
+// The altitude along a ray is r^2 - R^2 = q + (b + t)^2 in FP32: near the
+// horizontal (small b, the grazing rays) the terms have one sign and the
+// altitude is good to millimetres; for a steep ray q ~ -b^2 and the sum
+// cancels to ~0.4 m of altitude, the order of the reference's FP32
+// (r-R)(r+R) at sky_trace.wgsl:214.  Roots and shell crossings use the FP64 q.
 //
 // FP32 transcendentals, reciprocals and roots use the gfx950 hardware
 // instructions (v_exp/v_log/v_sin/v_cos/v_rcp/v_rsq/v_sqrt_f32, about 1 ulp)
