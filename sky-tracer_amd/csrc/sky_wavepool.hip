@@ -117,14 +117,13 @@ struct DSig {
     float aa5, aa6, aa8;  // sigma_Aa: d/d beta_Aa, d/d H_A, d/d angstrom
 };
 
-__device__ __forceinline__ DSig dsig(const DevScene &s, const AdjArgs &A, uint32_t band, float h, float bo,
-                                     const Coeffs &co)
+// rr, ra: the Rayleigh and aerosol density factors exp(-h / H) at h.
+__device__ __forceinline__ DSig dsig_x(const DevScene &s, const AdjArgs &A, uint32_t band, float h, float bo,
+                                       const Coeffs &co, float rr, float ra)
 {
     DSig d{};
     if (h < 0.0f || h > s.topf) return d;
     const float4 u = A.lam_units[band];
-    float rr = f_exp2(-h * s.l2_hr);
-    float ra = f_exp2(-h * s.l2_ha);
     float dh = h - s.hc;
     float tent = fmaxf(1.0f - fabsf(dh) * s.inv_hw, 0.0f);
     d.r0 = u.x * rr;
@@ -138,6 +137,12 @@ __device__ __forceinline__ DSig dsig(const DevScene &s, const AdjArgs &A, uint32
     d.as8 = -co.sas * u.w;
     d.aa8 = -co.saa * u.w;
     return d;
+}
+
+__device__ __forceinline__ DSig dsig(const DevScene &s, const AdjArgs &A, uint32_t band, float h, float bo,
+                                     const Coeffs &co)
+{
+    return dsig_x(s, A, band, h, bo, co, f_exp2(-h * s.l2_hr), f_exp2(-h * s.l2_ha));
 }
 
 // v += f * d sigma_t
@@ -452,7 +457,22 @@ __device__ __forceinline__ void track_step_fast(const DevScene &s, const MajSrc 
     const bool rt = RATIO && c.mode == kTrackRatio;
     // (an ended lane evaluates the medium at its end point, unused)
     const float h = lane_altitude(s, c, col ? tn : c.t);
-    const Coeffs co = lane_medium<true>(s, m, c, h);
+    Coeffs co;
+    float rr = 0.0f, ra = 0.0f;
+    if constexpr (ADJ) {
+        // the replays run the analytic medium only (the adjoint refuses
+        // tables): medium4's arithmetic, keeping its two density factors for
+        // the derivatives below
+        const v4f b = m.lam4[c.band];
+        rr = f_exp2(-h * s.l2_hr);
+        ra = f_exp2(-h * s.l2_ha);
+        float ro = 1.0f - fabsf(h - s.hc) * s.inv_hw;
+        ro = ro < 0.0f ? 0.0f : ro;
+        co = (h < 0.0f || h > s.topf) ? Coeffs{0.0f, 0.0f, 0.0f, 0.0f}
+                                      : Coeffs{b.x * rr, b.z * ra, b.y * ro, b.w * ra};
+    } else {
+        co = lane_medium<true>(s, m, c, h);
+    }
     const float st = co.sr + co.sas + co.so + co.saa;
     const float ratio = fminf(fmaxf(st * rm, 0.0f), 1.0f);
     const float u = rng_peek(c.rng, 0);
@@ -475,7 +495,7 @@ __device__ __forceinline__ void track_step_fast(const DevScene &s, const MajSrc 
     if constexpr (ADJ) {
         // accepted main-ray collision: d log(sigma_s / mu) into S; null
         // collision: d log(1 - sigma_t / mu) into S (main ray) or SS (shadow)
-        const DSig d = dsig(s, A, c.band, h, m.lam4[c.band].y, co);
+        const DSig d = dsig_x(s, A, c.band, h, m.lam4[c.band].y, co, rr, ra);
         const bool rej = col && !acc;
         const float scat = co.sr + co.sas;
         const float fs = vert && scat > 0.0f ? 1.0f / scat : 0.0f;
