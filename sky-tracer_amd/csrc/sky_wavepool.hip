@@ -117,13 +117,13 @@ struct DSig {
     float aa5, aa6, aa8;  // sigma_Aa: d/d beta_Aa, d/d H_A, d/d angstrom
 };
 
-// rr, ra: the Rayleigh and aerosol density factors exp(-h / H) at h.
-__device__ __forceinline__ DSig dsig_x(const DevScene &s, const AdjArgs &A, uint32_t band, float h, float bo,
+// u: the band's A.lam_units entry; rr, ra: the Rayleigh and aerosol density
+// factors exp(-h / H) at h.
+__device__ __forceinline__ DSig dsig_x(const DevScene &s, const AdjArgs &A, const float4 u, float h, float bo,
                                        const Coeffs &co, float rr, float ra)
 {
     DSig d{};
     if (h < 0.0f || h > s.topf) return d;
-    const float4 u = A.lam_units[band];
     float dh = h - s.hc;
     float tent = fmaxf(1.0f - fabsf(dh) * s.inv_hw, 0.0f);
     d.r0 = u.x * rr;
@@ -142,7 +142,7 @@ __device__ __forceinline__ DSig dsig_x(const DevScene &s, const AdjArgs &A, uint
 __device__ __forceinline__ DSig dsig(const DevScene &s, const AdjArgs &A, uint32_t band, float h, float bo,
                                      const Coeffs &co)
 {
-    return dsig_x(s, A, band, h, bo, co, f_exp2(-h * s.l2_hr), f_exp2(-h * s.l2_ha));
+    return dsig_x(s, A, A.lam_units[band], h, bo, co, f_exp2(-h * s.l2_hr), f_exp2(-h * s.l2_ha));
 }
 
 // v += f * d sigma_t
@@ -169,12 +169,13 @@ __device__ __forceinline__ void add_dsigma_s(float *v, const DSig &d, float f)
 }
 
 // v += d log p(mu), p = (sigma_R p_R + sigma_As p_HG) / sigma_s at the vertex.
+// u: the band's A.lam_units entry.
 __device__ __forceinline__ void add_dlog_phase(const DevScene &s, const AdjArgs &A, const AdjLane &a,
-                                               uint32_t band, float mu, float bo, float *v)
+                                               const float4 u, float mu, float bo, float *v)
 {
     Coeffs co{a.vsr, a.vsas, 0.0f, 0.0f};
     // the aerosol absorption share of sigma_As's partials equals sigma_As's
-    DSig d = dsig(s, A, band, a.vh, bo, co);
+    DSig d = dsig_x(s, A, u, a.vh, bo, co, f_exp2(-a.vh * s.l2_hr), f_exp2(-a.vh * s.l2_ha));
     float pr = phase_rayleigh(mu), ph = phase_hg(s.g, mu);
     float num = a.vsr * pr + a.vsas * ph;
     float scat = a.vsr + a.vsas;
@@ -228,6 +229,14 @@ struct MajSrc {
     // of being carried in registers by every lane
     LdsFloat4Ptr lam4;
     LdsFloatPtr lsun;
+    // FAST replays: the job's A.lam_units, an LDS copy (a derivative needs it
+    // at every tracking step)
+    LdsFloat4Ptr units;
+    __device__ __forceinline__ float4 unit(uint32_t band) const
+    {
+        const v4f u = units[band];
+        return make_float4(u.x, u.y, u.z, u.w);
+    }
     // FAST kernels: 1 / |d| (FP64) of each lane's current ray, kept from its
     // set-up for the vertex update at the ray's end
     __attribute__((address_space(3))) double *inv;
@@ -495,7 +504,7 @@ __device__ __forceinline__ void track_step_fast(const DevScene &s, const MajSrc 
     if constexpr (ADJ) {
         // accepted main-ray collision: d log(sigma_s / mu) into S; null
         // collision: d log(1 - sigma_t / mu) into S (main ray) or SS (shadow)
-        const DSig d = dsig_x(s, A, c.band, h, m.lam4[c.band].y, co, rr, ra);
+        const DSig d = dsig_x(s, A, m.unit(c.band), h, m.lam4[c.band].y, co, rr, ra);
         const bool rej = col && !acc;
         const float scat = co.sr + co.sas;
         const float fs = vert && scat > 0.0f ? 1.0f / scat : 0.0f;
@@ -675,7 +684,7 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
             need_ray = true;
             if constexpr (ADJ) {
                 for (int k = 0; k < kNP; ++k) a.SS[k] = a.S[k];
-                add_dlog_phase(s, A, a, c.band, mu, c.bo, a.SS);
+                add_dlog_phase(s, A, a, A.lam_units[c.band], mu, c.bo, a.SS);
             }
         }
         mode = (s.flags & kFlagRatio) ? kTrackRatio : kTrackDelta;
@@ -694,7 +703,7 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
             float mu = dot3(c.vin, dir);
             w = phase_is ? 1.0f
                          : (c.rw * phase_rayleigh(mu) + (1.0f - c.rw) * phase_hg(s.g, mu)) * (1.0f / kIsotropicPdf);
-            if constexpr (ADJ) add_dlog_phase(s, A, a, c.band, mu, c.bo, a.S);
+            if constexpr (ADJ) add_dlog_phase(s, A, a, A.lam_units[c.band], mu, c.bo, a.S);
         } else {
             // albedo/pi cos / (cos/pi)
             float co_out = fmaxf(dot3(nrm, dir), 0.0f);
@@ -890,7 +899,7 @@ __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSr
         // the same terms to the path's score
         float D[kNP];
         for (int k = 0; k < kNP; ++k) D[k] = 0.0f;
-        add_dlog_phase(s, A, a, c.band, mu_v, m.lam4[c.band].y, D);
+        add_dlog_phase(s, A, a, m.unit(c.band), mu_v, m.lam4[c.band].y, D);
         const float alb = s.albedo > 0.0f ? 1.0f / s.albedo : 0.0f;
         for (int k = 0; k < kNP; ++k) {
             const float t = scat_v ? D[k] : (k == 9 ? alb : 0.0f);
@@ -1172,6 +1181,8 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
     // FAST: [band] 4-float coefficients, then [band] sun radiance (kMaxLambda bands each)
     v4f *lds_lam4 = (v4f *)(lds_maj + a.maj_lds + 2 * a.shell_lds);
     float *lds_lsun = (float *)(lds_lam4 + (uint32_t)kMaxLambda);
+    // FAST replays: [band] A.lam_units after the sun radiance
+    v4f *lds_units = (v4f *)(lds_lsun + (uint32_t)kMaxLambda);
     uint32_t wd = 0;  // a watchdog fired on this lane
     uint32_t *diag = nullptr;
     LaneCtx c;
@@ -1223,6 +1234,7 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
         msrc.shells.use_lds = msrc.use_lds;
         msrc.lam4 = (LdsFloat4Ptr)lds_lam4;
         msrc.lsun = (LdsFloatPtr)lds_lsun;
+        msrc.units = (LdsFloat4Ptr)lds_units;
         msrc.inv = (__attribute__((address_space(3))) double *)lds_inv;
         if (msrc.use_lds) {
             if (loaded != jid) {
@@ -1240,6 +1252,10 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
                         const DevLambda l = s.lam[i];
                         lds_lam4[i] = v4f{l.br, l.bo, l.bas, l.baa};
                         lds_lsun[i] = l.lsun;
+                        if constexpr (ADJ) {
+                            const float4 u = A.lam_units[i];
+                            lds_units[i] = v4f{u.x, u.y, u.z, u.w};
+                        }
                     }
                 }
                 loaded = jid;
@@ -1608,11 +1624,14 @@ hipError_t launch_adjoint(const DevJob *d_jobs, const WavePlan &p, uint32_t *met
     a.track_steps = p.track_steps;
     a.maj_lds = p.maj_lds;
     a.shell_lds = p.shell_lds;
-    a.lam_lds = p.lam_lds;
+    // the FAST replays keep A.lam_units in LDS beside the per-band coefficients
+    const uint32_t units_lds = p.fast ? 4u * (uint32_t)kMaxLambda : 0u;
+    a.lam_lds = p.lam_lds + units_lds;
+    const uint32_t lds_bytes = p.lds_bytes + kWavesPerBlock * units_lds * (uint32_t)sizeof(float);
     typedef void (*AdjKernel)(PoolArgs, AdjArgs);
     static const AdjKernel kernels[4] = {k_wavepool_adjoint, k_wavepool_adjoint_fast, k_wavepool_adjoint_fastref,
                                          k_wavepool_adjoint_fastrt};
-    hipLaunchKernelGGL(kernels[p.fast], dim3(p.grid), dim3(p.block), p.lds_bytes, stream, a, A);
+    hipLaunchKernelGGL(kernels[p.fast], dim3(p.grid), dim3(p.block), lds_bytes, stream, a, A);
     return hipGetLastError();
 }
 
