@@ -227,7 +227,7 @@ def bench_batch(args, ctx):
         "value": value,
         "unit": "path x lambda samples/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 ray set-up)",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 only at grazing ground intersections)",
         "data": ("the reference's scene tables (data/*.csv), HG aerosol phase"
                  if args.workload == "skyview" and args.medium == "tables" else "synthetic (analytic atmosphere)"),
         "config": {"workload": what, "estimator": args.mode if args.workload == "skyview" else "carried_is",
@@ -294,7 +294,7 @@ def bench_adjoint(args, ctx):
         "value": evals_1pct,
         "unit": "gradient evaluations/s",
         "n_gpus": 1, "steps": 8, "warmup": max(args.warmup, 1), "ms_per_step": t_eval * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 ray set-up)",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 only at grazing ground intersections)",
         "data": "synthetic (analytic atmosphere)",
         "config": {"workload": f"c5: 256x128, 32 lambda, {spp} spp, depth<=8, sun 10 deg, 10-parameter adjoint",
                    "estimator": args.mode, "majorant_layers": base.majorant_layers,
@@ -635,7 +635,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (f64 ray set-up)",
+        "dtype": "f32 (f64 only at grazing ground intersections)",
         "data": "synthetic (analytic atmosphere, no external data)",
         "config": {"workload": f"{args.workload}: {w['width']}x{w['height']}, {w['n_lambda']} lambda, "
                                f"{w['spp']} spp, depth<={w['max_depth']}, sun {w['sun']} deg",

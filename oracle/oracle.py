@@ -97,6 +97,7 @@ class Counters(C.Structure):
             "shadow_rays",
             "rng_draws",
             "watchdog",
+            "graze64",
         )
     ]
 
@@ -153,6 +154,8 @@ def lib():
             fn = getattr(L, f"orc_trace_path_{s}")
             fn.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(Counters)]
             fn.restype = C.c_double
+        L.orc_ray_setup_f32.argtypes = [P, dp, dp, C.c_double, dp]
+        L.orc_ray_setup_f32.restype = C.c_int
         L.orc_transmittance_mc_f32.argtypes = [P, C.c_uint32, dp, dp, C.c_uint32, C.c_int]
         L.orc_transmittance_mc_f32.restype = C.c_double
         for name, args in (("orc_sample_rayleigh_mu", [C.c_double]), ("orc_sample_hg_mu", [C.c_double] * 2),
@@ -438,6 +441,23 @@ def trace_path(p: Params, x: int, y: int, sample: int, band: int, precision: str
     fn = getattr(lib(), f"orc_trace_path_{precision}")
     L = fn(C.byref(cp), x, y, sample, band, C.byref(cnt))
     return float(L), cnt.as_dict()
+
+
+def ray_setup_f32(p: Params, o, d, e: float = 0.0):
+    """The FP32 tracer's ray set-up: (dict of hit, kind, t_max, layer, b, q,
+    w_o; whether the FP64 grazing branch ran)."""
+    cp = p.to_c()
+    o = np.ascontiguousarray(o, dtype=np.float64)
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    out = np.zeros(7)
+    g = lib().orc_ray_setup_f32(C.byref(cp), _dp(o), _dp(d), float(e), _dp(out))
+    if g < 0:
+        raise ValueError("invalid configuration")
+    keys = ("hit", "kind", "t_max", "layer", "b", "q", "wo")
+    r = dict(zip(keys, out.tolist()))
+    for k in ("hit", "kind", "layer"):
+        r[k] = int(r[k])
+    return r, bool(g)
 
 
 def transmittance_mc(p: Params, band: int, o, d, n: int, ratio: bool) -> float:

@@ -759,18 +759,42 @@ static void add_counters(orc_counters *dst, const orc_counters *src)
 #define REAL float
 #define FN(name) name##_f32
 #define RF(x) x##f
+#define ORC_GRAZE64 1
+#define RFMA(a, b, c) fmaf(a, b, c) /* the GPU's fused multiply-adds of the geometry */
+#include "sky_oracle_trace.inc"
+#undef REAL
+#undef FN
+#undef RF
+#undef ORC_GRAZE64
+#undef RFMA
+
+#define REAL double
+#define FN(name) name##_f64
+#define RF(x) x
+#define ORC_GRAZE64 0
+#define RFMA(a, b, c) ((a) * (b) + (c))
 #include "sky_oracle_trace.inc"
 #undef REAL
 #undef FN
 #undef RF
 
-#define REAL double
-#define FN(name) name##_f64
-#define RF(x) x
-#include "sky_oracle_trace.inc"
-#undef REAL
-#undef FN
-#undef RF
+int orc_ray_setup_f32(const orc_params *p, const double o[3], const double d[3], double e, double out[7])
+{
+    scene_f32 S;
+    if (scene_init_f32(p, &S)) return -1;
+    orc_counters c;
+    memset(&c, 0, sizeof(c));
+    float of[3] = {(float)o[0], (float)o[1], (float)o[2]};
+    float df[3] = {(float)d[0], (float)d[1], (float)d[2]};
+    float w = (float)(o[0] * o[0] + o[1] * o[1] + o[2] * o[2] - S.R * S.R);
+    ray_f32 r;
+    memset(&r, 0, sizeof(r));
+    ray_setup_f32(&S, of, w, df, (float)e, &r, &c);
+    double v[7] = {r.hit, r.kind, r.t_max, r.layer, r.bf, r.qf, r.wo};
+    memcpy(out, v, sizeof(v));
+    scene_free_f32(&S);
+    return c.graze64 ? 1 : 0;
+}
 
 /* Statistical tracking estimator on one ray (tests of E[T] = exp(-tau)). */
 double orc_transmittance_mc_f32(const orc_params *p, uint32_t band, const double o[3],
@@ -781,8 +805,10 @@ double orc_transmittance_mc_f32(const orc_params *p, uint32_t band, const double
     orc_counters c;
     memset(&c, 0, sizeof(c));
     float df[3] = {(float)d[0], (float)d[1], (float)d[2]};
+    float of[3] = {(float)o[0], (float)o[1], (float)o[2]};
+    float w = (float)(o[0] * o[0] + o[1] * o[1] + o[2] * o[2] - S.R * S.R);
     ray_f32 r;
-    ray_setup_f32(&S, o, df, &r, &c);
+    ray_setup_f32(&S, of, w, df, 0.0f, &r, &c);
     double sum = 0.0;
     int wd = 0;
     for (uint32_t i = 0; i < n; ++i) {

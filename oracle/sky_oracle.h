@@ -48,6 +48,7 @@ typedef struct orc_counters {
     uint64_t shadow_rays;      /* shadow rays traced                        */
     uint64_t rng_draws;        /* pcg4d evaluations                         */
     uint64_t watchdog;         /* paths that tripped a watchdog             */
+    uint64_t graze64;          /* FP32 ray set-ups formed in FP64 (ground-grazing rays) */
 } orc_counters;
 
 /* The reference's tabulated SceneData (data.rs:53-152): profiles in
@@ -183,6 +184,12 @@ double orc_trace_path_f64(const orc_params *p, uint32_t x, uint32_t y, uint32_t 
 /* Tracking estimators on one ray, for statistical known-answer tests.
  * Returns the mean over n keys of the escape indicator (delta tracking,
  * sky_trace.wgsl:441-475) or of the ratio-tracking weight. */
+/* The FP32 tracer's ray set-up from the vertex o (rounded to FP32, its
+ * r^2 - R^2 formed in double) along d with the origin moved by e:
+ * out = {hit, kind, t_max, layer, b, q, w_o}.  Returns 1 where the set-up
+ * took the FP64 grazing branch, 0 otherwise, -1 on a bad configuration. */
+int orc_ray_setup_f32(const orc_params *p, const double o[3], const double d[3], double e, double out[7]);
+
 double orc_transmittance_mc_f32(const orc_params *p, uint32_t band, const double o[3],
                                 const double d[3], uint32_t n, int ratio);
 

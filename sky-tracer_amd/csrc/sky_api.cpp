@@ -132,7 +132,7 @@ bool carve_scratch(uint8_t *base, const sky::HostScene &hs, size_t accum_bytes, 
     b.job = take(sizeof(sky::DevJob));
     b.lam = take(hs.lam.size() * sizeof(sky::DevLambda));
     b.maj = take(hs.maj.size() * sizeof(float));
-    b.shells = take(hs.shell_d.size() * sizeof(double));
+    b.shells = take(hs.shell_w.size() * sizeof(float));
     b.tab = tab_bytes ? take(tab_bytes) : nullptr;
     b.accum = accum_bytes ? take(accum_bytes) : nullptr;
     return off <= SKY_NOALLOC_SCRATCH_BYTES;
@@ -171,7 +171,7 @@ sky_status run_render(sky_context *ctx, sky::HostScene &hs, hipStream_t stream, 
     } else {
         HIP_TRY(ctx->lam.ensure(hs.lam.size() * sizeof(sky::DevLambda)), "alloc lambda table", SKY_ERR_OOM);
         HIP_TRY(ctx->maj.ensure(hs.maj.size() * sizeof(float)), "alloc majorants", SKY_ERR_OOM);
-        HIP_TRY(ctx->shells.ensure(hs.shell_d.size() * sizeof(double)), "alloc shell table", SKY_ERR_OOM);
+        HIP_TRY(ctx->shells.ensure(hs.shell_w.size() * sizeof(float)), "alloc shell table", SKY_ERR_OOM);
         HIP_TRY(ctx->diag.ensure(64), "alloc diagnostics", SKY_ERR_OOM);
         HIP_TRY(ctx->meta.ensure(64), "alloc work counters", SKY_ERR_OOM);
         HIP_TRY(ctx->jobs.ensure(sizeof(sky::DevJob)), "alloc job table", SKY_ERR_OOM);
@@ -195,7 +195,7 @@ sky_status run_render(sky_context *ctx, sky::HostScene &hs, hipStream_t stream, 
             "upload lambda table", SKY_ERR_COPY);
     HIP_TRY(hipMemcpyAsync(b.maj, hs.maj.data(), hs.maj.size() * sizeof(float), hipMemcpyHostToDevice, stream),
             "upload majorants", SKY_ERR_COPY);
-    HIP_TRY(hipMemcpyAsync(b.shells, hs.shell_d.data(), hs.shell_d.size() * sizeof(double), hipMemcpyHostToDevice,
+    HIP_TRY(hipMemcpyAsync(b.shells, hs.shell_w.data(), hs.shell_w.size() * sizeof(float), hipMemcpyHostToDevice,
                            stream),
             "upload shell table", SKY_ERR_COPY);
     HIP_TRY(sky::launch_zero((uint32_t *)b.diag, 16, stream), "clear diagnostics", SKY_ERR_SYNC);
@@ -205,7 +205,7 @@ sky_status run_render(sky_context *ctx, sky::HostScene &hs, hipStream_t stream, 
     hs.tab.attach(s, b.tab);
     s.lam = (const sky::DevLambda *)b.lam;
     s.maj = (const float *)b.maj;
-    s.shell_d = (const double *)b.shells;
+    s.shell_w = (const float *)b.shells;
     s.diag = (uint32_t *)b.diag;
     sky::LaunchShape shape;
     sky::WavePlan plan;
@@ -518,7 +518,7 @@ sky_status adjoint_impl(sky_context *ctx, const sky_planet *planet, const sky_at
     sky::DevScene &s = hs.dev;
     HIP_TRY(ctx->lam.ensure(hs.lam.size() * sizeof(sky::DevLambda)), "alloc lambda table", SKY_ERR_OOM);
     HIP_TRY(ctx->maj.ensure(hs.maj.size() * sizeof(float)), "alloc majorants", SKY_ERR_OOM);
-    HIP_TRY(ctx->shells.ensure(hs.shell_d.size() * sizeof(double)), "alloc shell table", SKY_ERR_OOM);
+    HIP_TRY(ctx->shells.ensure(hs.shell_w.size() * sizeof(float)), "alloc shell table", SKY_ERR_OOM);
     HIP_TRY(ctx->diag.ensure(64), "alloc diagnostics", SKY_ERR_OOM);
     HIP_TRY(ctx->meta.ensure(64), "alloc work counters", SKY_ERR_OOM);
     HIP_TRY(ctx->jobs.ensure(sizeof(sky::DevJob)), "alloc job table", SKY_ERR_OOM);
@@ -534,7 +534,7 @@ sky_status adjoint_impl(sky_context *ctx, const sky_planet *planet, const sky_at
                            hipMemcpyHostToDevice, stream), "upload lambda table", SKY_ERR_COPY);
     HIP_TRY(hipMemcpyAsync(ctx->maj.p, hs.maj.data(), hs.maj.size() * sizeof(float), hipMemcpyHostToDevice,
                            stream), "upload majorants", SKY_ERR_COPY);
-    HIP_TRY(hipMemcpyAsync(ctx->shells.p, hs.shell_d.data(), hs.shell_d.size() * sizeof(double),
+    HIP_TRY(hipMemcpyAsync(ctx->shells.p, hs.shell_w.data(), hs.shell_w.size() * sizeof(float),
                            hipMemcpyHostToDevice, stream), "upload shell table", SKY_ERR_COPY);
     HIP_TRY(hipMemcpyAsync(ctx->adj_units.p, units.data(), units.size() * sizeof(float4), hipMemcpyHostToDevice,
                            stream), "upload adjoint table", SKY_ERR_COPY);
@@ -543,7 +543,7 @@ sky_status adjoint_impl(sky_context *ctx, const sky_planet *planet, const sky_at
             SKY_ERR_SYNC);
     s.lam = (const sky::DevLambda *)ctx->lam.p;
     s.maj = (const float *)ctx->maj.p;
-    s.shell_d = (const double *)ctx->shells.p;
+    s.shell_w = (const float *)ctx->shells.p;
     s.diag = (uint32_t *)ctx->diag.p;
     s.film = s.film_m2 = s.xyz = nullptr;
     ctx->host_jobs.assign(1, sky::DevJob{});
@@ -681,7 +681,7 @@ sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint3
         if (!h.tab.empty()) n_tab += h.tab.bytes();
         n_lam += h.lam.size();
         n_maj += h.maj.size();
-        n_shl += h.shell_d.size();
+        n_shl += h.shell_w.size();
         n_xyz += (size_t)npx * 3;
         if (items[i].film) n_film += (size_t)npx * h.nlam;
         sky::DevJob &j = ctx->host_jobs[k];
@@ -701,7 +701,7 @@ sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint3
     const uint32_t meta_words = 8 + 4 * n;
     HIP_TRY(ctx->b_lam.ensure(n_lam * sizeof(sky::DevLambda)), "alloc batch lambda tables", SKY_ERR_OOM);
     HIP_TRY(ctx->b_maj.ensure(n_maj * sizeof(float)), "alloc batch majorants", SKY_ERR_OOM);
-    HIP_TRY(ctx->b_shells.ensure(n_shl * sizeof(double)), "alloc batch shell tables", SKY_ERR_OOM);
+    HIP_TRY(ctx->b_shells.ensure(n_shl * sizeof(float)), "alloc batch shell tables", SKY_ERR_OOM);
     HIP_TRY(ctx->b_xyz.ensure(n_xyz * sizeof(float)), "alloc batch xyz", SKY_ERR_OOM);
     if (n_film) HIP_TRY(ctx->b_film.ensure(n_film * sizeof(float)), "alloc batch film", SKY_ERR_OOM);
     HIP_TRY(ctx->b_accum.ensure(n_acc * sizeof(float2)), "alloc batch partial sums", SKY_ERR_OOM);
@@ -712,18 +712,18 @@ sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint3
     HIP_TRY(ctx->diag.ensure(64), "alloc diagnostics", SKY_ERR_OOM);
     std::vector<sky::DevLambda> all_lam(n_lam);
     std::vector<float> all_maj(n_maj);
-    std::vector<double> all_shl(n_shl);
+    std::vector<float> all_shl(n_shl);
     uint32_t *meta = (uint32_t *)ctx->b_meta.p;
     for (uint32_t k = 0; k < n; ++k) {
         uint32_t i = order[k];
         sky::HostScene &h = scenes[i];
         std::copy(h.lam.begin(), h.lam.end(), all_lam.begin() + o_lam[i]);
         std::copy(h.maj.begin(), h.maj.end(), all_maj.begin() + o_maj[i]);
-        std::copy(h.shell_d.begin(), h.shell_d.end(), all_shl.begin() + o_shl[i]);
+        std::copy(h.shell_w.begin(), h.shell_w.end(), all_shl.begin() + o_shl[i]);
         sky::DevJob &j = ctx->host_jobs[k];
         j.s.lam = (const sky::DevLambda *)ctx->b_lam.p + o_lam[i];
         j.s.maj = (const float *)ctx->b_maj.p + o_maj[i];
-        j.s.shell_d = (const double *)ctx->b_shells.p + o_shl[i];
+        j.s.shell_w = (const float *)ctx->b_shells.p + o_shl[i];
         if (!h.tab.empty()) {
             std::vector<uint8_t> blob;
             h.tab.pack(blob);
@@ -742,7 +742,7 @@ sky_status sky_render_batch(sky_context *ctx, const sky_batch_item *items, uint3
                            stream), "upload batch lambda tables", SKY_ERR_COPY);
     HIP_TRY(hipMemcpyAsync(ctx->b_maj.p, all_maj.data(), n_maj * sizeof(float), hipMemcpyHostToDevice, stream),
             "upload batch majorants", SKY_ERR_COPY);
-    HIP_TRY(hipMemcpyAsync(ctx->b_shells.p, all_shl.data(), n_shl * sizeof(double), hipMemcpyHostToDevice, stream),
+    HIP_TRY(hipMemcpyAsync(ctx->b_shells.p, all_shl.data(), n_shl * sizeof(float), hipMemcpyHostToDevice, stream),
             "upload batch shell tables", SKY_ERR_COPY);
     HIP_TRY(hipMemcpyAsync(ctx->b_jobs.p, ctx->host_jobs.data(), n * sizeof(sky::DevJob), hipMemcpyHostToDevice,
                            stream), "upload batch job table", SKY_ERR_COPY);

@@ -2,13 +2,16 @@
 // for gfx950 (CDNA4, wave64).  Shared by the wave-pool integrator, the simple
 // per-(pixel, lambda) integrator and the optical-depth kernel.
 //
-// Precision split: ray set-up (origin, b = o.d, q = |o|^2 - b^2 - R^2, boundary
-// distances, shell discriminants) is FP64; every per-step quantity is FP32.
-// The altitude along a ray is r^2 - R^2 = q + (b + t)^2 in FP32: near the
-// horizontal (small b, the grazing rays) the terms have one sign and the
-// altitude is good to millimetres; for a steep ray q ~ -b^2 and the sum
-// cancels to ~0.4 m of altitude, the order of the reference's FP32
-// (r-R)(r+R) at sky_trace.wgsl:214.  Roots and shell crossings use the FP64 q.
+// Precision: geometry is FP32 and never forms r^2 - R^2 by subtracting R^2
+// from r^2 (which cancels at 6360 km, the reference's FP32 (r-R)(r+R) at
+// sky_trace.wgsl:214 is good to ~0.5 m).  Each vertex carries w = r^2 - R^2
+// itself; along a ray from an origin with w_o and b = o.d it is
+// w(t) = w_o + t (2b + t), good to ~1 cm of altitude for every ray, and set
+// to exactly 0 at a ground hit.  Roots come from q = w - b^2 (the same for
+// every point of a line), formed with one rounding.  FP64 is used only where
+// the ground's discriminant -q lies within the FP32 error of b of zero: a ray
+// grazing the ground within centimetres, whose hit or miss that error could
+// flip (ray_setup).
 //
 // FP32 transcendentals, reciprocals and roots use the gfx950 hardware
 // instructions (v_exp/v_log/v_sin/v_cos/v_rcp/v_rsq/v_sqrt_f32, about 1 ulp)
@@ -43,8 +46,10 @@ struct DevLambda {
 
 // Everything one render needs on the device.
 struct DevScene {
-    double R, top, dz, delta_top, obs_y;
-    double shell_tol;  // r^2 - R^2 within which an inward origin counts as on a shell boundary
+    double R, top, dz, obs_y;  // host-side and optical-depth-kernel geometry (FP64)
+    float delta_topf;  // r_top^2 - R^2
+    float shell_tolf;  // r^2 - R^2 within which an inward origin counts as on a shell boundary
+    float obs_yf, obs_wf;  // camera at (0, obs_yf, 0), its r^2 - R^2
     // Shell k's lower boundary h_k (k = 0..L, h_L = top): uniform, or with
     // thicknesses growing by a factor g.  First guess of the shell of
     // altitude h: floor(h c2) (uniform, c1 = 0) or floor(log2(1 + h c1) c2).
@@ -81,7 +86,7 @@ struct DevScene {
     float tab_cell_inv_dh;
     const DevLambda *lam;
     const float *maj;  // [lambda * L + layer]
-    const double *shell_d;  // [L + 1]: r_k^2 - R^2 = h_k (2R + h_k)
+    const float *shell_w;  // [L + 1]: r_k^2 - R^2 = h_k (2R + h_k)
     float *film;       // [pixel * nlam + lambda] (may be null)
     float *film_m2;    // same layout (may be null)
     float *xyz;        // [pixel * 3 + c] (may be null)
@@ -225,81 +230,107 @@ __device__ __forceinline__ f3 frame_dir(f3 n, float mu, float st, float phi_rev)
 }
 
 // ------------------------------------------------------------- rays
-typedef __attribute__((address_space(3))) const double *LdsDoublePtr;
-typedef __attribute__((address_space(1))) const double *GlobalDoublePtr;
+typedef __attribute__((address_space(3))) const float *LdsShellPtr;
+typedef __attribute__((address_space(1))) const float *GlobalShellPtr;
 
 // The shell table r_k^2 - R^2 (k = 0..L) read from LDS or global memory; the
 // typed pointers keep each read a plain LDS or global load.
 struct ShellTab {
-    LdsDoublePtr lds;
-    GlobalDoublePtr glob;
+    LdsShellPtr lds;
+    GlobalShellPtr glob;
     bool use_lds;
-    __device__ __forceinline__ double operator[](int k) const { return use_lds ? lds[k] : glob[k]; }
+    __device__ __forceinline__ float operator[](int k) const { return use_lds ? lds[k] : glob[k]; }
 };
 
 __device__ __forceinline__ ShellTab global_shells(const DevScene &s)
 {
     ShellTab t;
-    t.lds = (LdsDoublePtr) nullptr;
-    t.glob = (GlobalDoublePtr)s.shell_d;
+    t.lds = (LdsShellPtr) nullptr;
+    t.glob = (GlobalShellPtr)s.shell_w;
     t.use_lds = false;
     return t;
 }
 
 struct Ray {
-    double ox, oy, oz;  // origin (km), FP64
-    double b, q;        // o.d and |o|^2 - (o.d)^2 - R^2
-    double inv;         // 1 / |d| (FP64) of the FP32 direction
-    f3 d;               // unit direction
-    float bf, qf, tmax;
+    f3 d;          // unit direction
+    float bf, qf;  // b = o.d at the origin o, q = w - b^2 (the line's; origin-independent)
+    float wo;      // r^2 - R^2 at the origin
+    float tmax;
     int kind;     // 0 ground, 1 atmosphere exit (sky_trace.wgsl:89-93)
     int layer;    // majorant shell at t = 0
     int outward;  // moving away from the planet centre at t = 0
     int hit;
 };
 
-// Ray set-up: next_boundary (sky_trace.wgsl:233-250) from the impact
-// parameter -- b^2 - c of wgsl:212-215 is D - q with D = r_shell^2 - R^2 --
-// and the initial shell of the walk, found by comparing |o|^2 - R^2 with the
-// shell table (layer_for_altitude, wgsl:252-258, generalised to non-uniform
-// shells).  d is an FP32-normalised direction.
-template <int COUNT_L = 0>
-__device__ __forceinline__ void ray_setup(const DevScene &s, const ShellTab &tab, double ox, double oy,
-                                          double oz, f3 d, Ray &r)
+// r^2 - R^2 at distance t along a ray whose origin has b = o.d and w_o:
+// |o + t d|^2 - R^2 = w_o + t (2b + t).
+__device__ __forceinline__ float ray_w(float bf, float wo, float t)
 {
-    double dx = d.x, dy = d.y, dz = d.z;
-    double inv = inv_norm64(dx, dy, dz);
-    r.inv = inv;
-    r.ox = ox;
-    r.oy = oy;
-    r.oz = oz;
-    r.d = d;
-    double oo = ox * ox + oy * oy + oz * oz;
-    double b = (ox * dx + oy * dy + oz * dz) * inv;
-    double q = oo - b * b - s.R * s.R;
-    r.b = b;
-    r.q = q;
-    r.bf = (float)b;
-    r.qf = (float)q;
-    double disc_top = s.delta_top - q;
-    r.hit = disc_top >= 0.0;
-    const double eps = (double)kRayEpsilonKm;
-    double t_exit = -b + sqrt64(disc_top > 0.0 ? disc_top : 0.0);
-    double tg = -1.0;
-    if (-q >= 0.0) {
-        double sg = sqrt64(-q);
-        double t0 = -b - sg, t1 = -b + sg;
-        tg = t0 > eps ? t0 : (t1 > eps ? t1 : -1.0);
+    return __builtin_fmaf(t, __builtin_fmaf(2.0f, bf, t), wo);
+}
+
+// The FP32 error bound of q = w - b0^2 from the error of b0 = p.d: three
+// roundings of terms at most |p| <= 6460 km give |db0| <= 3 2^-24 |p| <= 1.2e-3
+// km, so |dq| <= 2 |b0| |db0| + b0^2 |(|d|^2 - 1)| (the FP32 unit direction)
+// + rounding, below 2.4e-3 |b0| + 2.4e-7 b0^2 + 1e-5.
+__device__ __forceinline__ float graze_tol(float b0)
+{
+    const float ab = fabsf(b0);
+    return __builtin_fmaf(ab, __builtin_fmaf(2.4e-7f, ab, 2.4e-3f), 1e-5f);
+}
+
+// Ray set-up: next_boundary (sky_trace.wgsl:233-250) and the initial shell of
+// the walk (layer_for_altitude, wgsl:252-258, generalised to non-uniform
+// shells) for the ray from p + e d along the unit direction d, where the
+// vertex p has r^2 - R^2 = w (e: the reference's ray epsilon, wgsl:744, 766,
+// or 0).  b^2 - c of wgsl:212-215 is -q, the same for every origin on the
+// line; the near ground root is the stable w_o / t1 (the roots' product).
+// Where -q is within its FP32 error of zero (graze_tol) b, q and the ground
+// roots are formed again in FP64 from the same p, w and d.
+template <int COUNT_L = 0>
+__device__ __forceinline__ void ray_setup(const DevScene &s, const ShellTab &tab, f3 p, float w, f3 d, float e,
+                                          Ray &r)
+{
+    const float b0 = __builtin_fmaf(p.z, d.z, __builtin_fmaf(p.y, d.y, p.x * d.x));
+    float q = __builtin_fmaf(-b0, b0, w);
+    float b = b0 + e;
+    const float wo = __builtin_fmaf(e, b0 + b, w);
+    const float eps = kRayEpsilonKm;
+    float tg = -1.0f;
+    if (fabsf(q) <= graze_tol(b0)) {
+        // grazing the ground: FP64 from the same p, w and (renormalised) d
+        const double dx = d.x, dy = d.y, dz = d.z;
+        const double b64 = fma((double)p.z, dz, fma((double)p.y, dy, (double)p.x * dx)) * inv_norm64(dx, dy, dz);
+        const double q64 = (double)w - b64 * b64;
+        const double be = b64 + (double)e;
+        if (q64 <= 0.0) {
+            const double sg = sqrt64(-q64);
+            const double t0 = -be - sg, t1 = sg - be;
+            tg = t0 > (double)eps ? (float)t0 : (t1 > (double)eps ? (float)t1 : -1.0f);
+        }
+        q = (float)q64;
+        b = (float)be;
+    } else if (q < 0.0f) {
+        const float sg = f_sqrt(-q);
+        const float t1 = sg - b;
+        const float t0 = b < 0.0f ? f_div(wo, t1) : -b - sg;
+        tg = t0 > eps ? t0 : (t1 > eps ? t1 : -1.0f);
     }
+    r.d = d;
+    r.bf = b;
+    r.qf = q;
+    r.wo = wo;
+    const float disc_top = s.delta_topf - q;
+    r.hit = disc_top >= 0.0f;
+    float t_exit = f_sqrt(fmaxf(disc_top, 0.0f)) - b;
     if (t_exit < eps) t_exit = eps;
     if (tg > eps && tg < t_exit) {
-        r.tmax = (float)tg;
+        r.tmax = tg;
         r.kind = 0;
     } else {
-        r.tmax = (float)t_exit;
+        r.tmax = t_exit;
         r.kind = 1;
     }
-    const double w0 = oo - s.R * s.R;
     const int L = (int)s.L;
     int k = 0;
     if constexpr (COUNT_L > 0) {
@@ -307,35 +338,36 @@ __device__ __forceinline__ void ray_setup(const DevScene &s, const ShellTab &tab
         // is the number of interior boundaries at or below it (L is the same
         // for the whole wave, so the loop runs exactly L - 1 times)
 #pragma unroll 1
-        for (int j = 1; j < L; ++j) k += tab[j] <= w0 ? 1 : 0;
+        for (int j = 1; j < L; ++j) k += tab[j] <= wo ? 1 : 0;
     } else {
-        const float wf = (float)w0;
-        const float h0 = f_div(wf, f_sqrt(fmaxf(s.R2f + wf, 0.0f)) + s.Rf);
+        const float h0 = f_div(wo, f_sqrt(fmaxf(s.R2f + wo, 0.0f)) + s.Rf);
         float kg = s.shell_c1 > 0.0f ? f_log2(fmaxf(1.0f + h0 * s.shell_c1, 1.0f)) * s.shell_c2 : h0 * s.shell_c2;
         k = (int)fminf(fmaxf(kg, 0.0f), (float)(L - 1));
-        while (k > 0 && tab[k] > w0) --k;
-        while (k < L - 1 && tab[k + 1] <= w0) ++k;
+        while (k > 0 && tab[k] > wo) --k;
+        while (k < L - 1 && tab[k + 1] <= wo) ++k;
     }
-    r.outward = b >= 0.0;
-    if (!r.outward && k > 0 && w0 - tab[k] <= s.shell_tol) k -= 1;
+    r.outward = b >= 0.0f;
+    if (!r.outward && k > 0 && wo - tab[k] <= s.shell_tolf) k -= 1;
     r.layer = k;
 }
 
 // Next crossing of the majorant-shell walk (the segments of
-// sky_trace.wgsl:284-294): D_k - q in FP64, its root in FP32.
-__device__ __forceinline__ float next_crossing(const DevScene &s, const ShellTab &tab, double q, float bf, int k,
+// sky_trace.wgsl:284-294): the root of D_k - q for the boundary the ray meets
+// next (an error e of D_k - q puts the crossing within e / 2R of the
+// boundary's altitude).
+__device__ __forceinline__ float next_crossing(const DevScene &s, const ShellTab &tab, float q, float bf, int k,
                                                int out, int &kn, int &on)
 {
     if (!out) {
-        double disc = tab[k] - q;
-        if (disc > 0.0) {
+        float disc = tab[k] - q;
+        if (disc > 0.0f) {
             on = 0;
             if (k == 0) {
                 kn = 0;
                 return __builtin_inff();
             }
             kn = k - 1;
-            return -f_sqrt((float)disc) - bf;
+            return -f_sqrt(disc) - bf;
         }
     }
     on = 1;
@@ -344,30 +376,28 @@ __device__ __forceinline__ float next_crossing(const DevScene &s, const ShellTab
         return __builtin_inff();
     }
     kn = k + 1;
-    double disc = tab[k + 1] - q;
-    return f_sqrt((float)(disc > 0.0 ? disc : 0.0)) - bf;
+    float disc = tab[k + 1] - q;
+    return f_sqrt(fmaxf(disc, 0.0f)) - bf;
 }
 
 // The same crossing without branches, for a shell table in LDS: both
 // boundaries of shell k are read at once and the root of the selected one is
-// taken, so every lane of a wave runs the same instructions.  Bit-identical
-// to next_crossing.
-__device__ __forceinline__ float next_crossing_sel(const DevScene &s, LdsDoublePtr tab, double q, float bf, int k,
+// taken, so every lane of a wave runs the same instructions.  Equal to
+// next_crossing.
+__device__ __forceinline__ float next_crossing_sel(const DevScene &s, LdsShellPtr tab, float q, float bf, int k,
                                                    int out, int &kn, int &on)
 {
-    const double din = tab[k];       // inner boundary of shell k
-    const double dout = tab[k + 1];  // outer boundary (the table has L + 1 entries)
+    const float din = tab[k];       // inner boundary of shell k
+    const float dout = tab[k + 1];  // outer boundary (the table has L + 1 entries)
     // inward iff the ray reaches the inner boundary: din - q > 0, which for
-    // finite operands holds exactly when din > q; then one subtraction for
-    // the chosen boundary (positive when inward)
+    // finite operands holds exactly when din > q
     const bool inw = !out && din > q;
     on = inw ? 0 : 1;
     // the next shell, clamped to [0, L - 1]: it stays k exactly at the two
     // edges (inward from shell 0, outward from the top shell)
     kn = min(max(inw ? k - 1 : k + 1, 0), (int)s.L - 1);
     const bool edge = kn == k;
-    const double d = (inw ? din : dout) - q;
-    const float r = f_sqrt((float)(d > 0.0 ? d : 0.0));
+    const float r = f_sqrt(fmaxf((inw ? din : dout) - q, 0.0f));
     const float tc = (inw ? -r : r) - bf;
     return edge ? __builtin_inff() : tc;
 }
@@ -375,20 +405,19 @@ __device__ __forceinline__ float next_crossing_sel(const DevScene &s, LdsDoubleP
 __device__ __forceinline__ float next_crossing(const DevScene &s, const ShellTab &tab, const Ray &r, int k,
                                                int out, int &kn, int &on)
 {
-    return next_crossing(s, tab, r.q, r.bf, k, out, kn, on);
+    return next_crossing(s, tab, r.qf, r.bf, k, out, kn, on);
 }
 
-// Altitude at distance t: (q + (b + t)^2) / (sqrt(R^2 + q + (b + t)^2) + R).
-__device__ __forceinline__ float altitude_bq(const DevScene &s, float bf, float qf, float t)
+// Altitude at distance t: w / (sqrt(R^2 + w) + R) with w = ray_w.
+__device__ __forceinline__ float altitude_bw(const DevScene &s, float bf, float wo, float t)
 {
-    float sv = bf + t;
-    float w = qf + sv * sv;
+    const float w = ray_w(bf, wo, t);
     return f_div(w, f_sqrt(s.R2f + w) + s.Rf);
 }
 
 __device__ __forceinline__ float altitude(const DevScene &s, const Ray &r, float t)
 {
-    return altitude_bq(s, r.bf, r.qf, t);
+    return altitude_bw(s, r.bf, r.wo, t);
 }
 
 // coefficients_at (sky_trace.wgsl:356-386) for the analytic profiles.

@@ -48,7 +48,7 @@ struct HostScene {
     DevScene dev{};
     std::vector<DevLambda> lam;
     std::vector<float> maj;
-    std::vector<double> shell_d;  // [L + 1] r_k^2 - R^2 of the shell boundaries
+    std::vector<float> shell_w;  // [L + 1] r_k^2 - R^2 of the shell boundaries
     MediumTable tab;              // empty: analytic medium
     uint32_t nlam = 0;
 };
@@ -91,7 +91,7 @@ struct PoolArgs {
     unsigned long long *t_start;
     uint32_t event_threshold, track_steps;
     uint32_t maj_lds;  // floats per wave of the LDS majorant copy (0: read the grids from global memory)
-    uint32_t shell_lds;  // doubles per wave of the LDS shell table
+    uint32_t shell_lds;  // floats per wave of the LDS shell table
     uint32_t lam_lds;    // floats per wave of the LDS per-band coefficients (FAST kernels)
 };
 
@@ -108,7 +108,7 @@ struct AdjArgs {
 struct WavePlan {
     uint32_t n_items = 0, grid = 0, block = 256, event_threshold = 32, track_steps = 8, occupancy = 0;
     uint32_t maj_lds = 0;   // floats per wave for the majorant grid in LDS (0: global)
-    uint32_t shell_lds = 0; // doubles per wave for the shell table in LDS (0 with maj_lds 0)
+    uint32_t shell_lds = 0; // floats per wave for the shell table in LDS (0 with maj_lds 0)
     uint32_t lds_bytes = 0; // dynamic LDS per block
     int fast = 0;           // 1: the _fast, 2: the _fastref, 3: the _fastrt kernels (see plan_launch)
     uint32_t lam_lds = 0;   // floats per wave for the per-band coefficients in LDS (fast kernels)

@@ -521,12 +521,12 @@ sky_status build_scene(const sky_planet *planet, const sky_atmosphere *a, const 
     s.top = planet->atmosphere_radius_km - planet->ground_radius_km;
     s.L = cfg->majorant_layers;
     s.dz = s.top / s.L;
-    out.shell_d.resize(s.L + 1);
+    out.shell_w.resize(s.L + 1);
     for (uint32_t k = 0; k <= s.L; ++k) {
         double h = shell_height(s.top, s.L, growth, k);
-        out.shell_d[k] = h * (2.0 * s.R + h);
+        out.shell_w[k] = (float)(h * (2.0 * s.R + h));
     }
-    s.shell_tol = 2e-9 * s.R;  // 1e-9 km of height
+    s.shell_tolf = (float)(2e-9 * s.R);  // 1e-9 km of height
     if (growth > 1.0) {
         s.shell_c1 = (float)((std::pow(growth, (double)s.L) - 1.0) / s.top);
         s.shell_c2 = (float)(1.0 / std::log2(growth));
@@ -534,8 +534,11 @@ sky_status build_scene(const sky_planet *planet, const sky_atmosphere *a, const 
         s.shell_c1 = 0.0f;
         s.shell_c2 = (float)(s.L / s.top);
     }
-    s.delta_top = (planet->atmosphere_radius_km - s.R) * (planet->atmosphere_radius_km + s.R);
-    s.obs_y = s.R + std::fmax(cfg->observer_altitude_km, 0.0);
+    s.delta_topf = (float)((planet->atmosphere_radius_km - s.R) * (planet->atmosphere_radius_km + s.R));
+    const double h_obs = std::fmax(cfg->observer_altitude_km, 0.0);
+    s.obs_y = s.R + h_obs;
+    s.obs_yf = (float)s.obs_y;
+    s.obs_wf = (float)(h_obs * (2.0 * s.R + h_obs));
     s.Rf = (float)s.R;
     s.R2f = (float)(s.R * s.R);
     s.topf = (float)s.top;

@@ -100,7 +100,8 @@ __device__ __forceinline__ bool roulette(const DevScene &s, float &tp, uint32_t 
 
 // trace_path (sky_trace.wgsl:703-776) with the camera ray of main
 // (wgsl:799-807), the depth cap, the roulette start depth and optional
-// phase importance sampling.  Positions of vertices are FP64.
+// phase importance sampling.  A vertex is its FP32 position p and its
+// r^2 - R^2 w (sky_device.hpp); new rays start the ray epsilon from it.
 __device__ float trace_path(const DevScene &s, uint32_t x, uint32_t y, uint32_t sample,
                             uint32_t band, bool &wd)
 {
@@ -111,7 +112,8 @@ __device__ float trace_path(const DevScene &s, uint32_t x, uint32_t y, uint32_t 
     float j1 = rng_next(rng);
     float j2 = rng_next(rng);
     f3 dir = camera_dir(s, x, y, j1, j2);
-    double ox = 0.0, oy = s.obs_y, oz = 0.0;
+    f3 p{0.0f, s.obs_yf, 0.0f};
+    float w = s.obs_wf;
     float tp = 1.0f, L = 0.0f;
     uint32_t depth = 0;
     const uint32_t n_light = s.dls ? s.dls : 1u;
@@ -122,17 +124,18 @@ __device__ float trace_path(const DevScene &s, uint32_t x, uint32_t y, uint32_t 
             return L;
         }
         Ray ray;
-        ray_setup(s, global_shells(s), ox, oy, oz, dir, ray);
+        const float e = depth > 0 ? kRayEpsilonKm : 0.0f;
+        ray_setup(s, global_shells(s), p, w, dir, e, ray);
+        // the main ray's origin becomes the vertex
+        p = f3{__builtin_fmaf(dir.x, e, p.x), __builtin_fmaf(dir.y, e, p.y), __builtin_fmaf(dir.z, e, p.z)};
+        w = ray.wo;
         if (!ray.hit) break;
         float th, Tdummy;
         Coeffs co;
         if (track(s, band, l, maj_row, ray, rng, kTrackCollision, th, co, Tdummy, wd)) {
-            // ray.d is FP32-normalised; positions use its FP64 unit vector.
-            double dx = ray.d.x, dy = ray.d.y, dz = ray.d.z;
-            double inv = inv_norm64(dx, dy, dz);
-            double px = ray.ox + dx * inv * (double)th;
-            double py = ray.oy + dy * inv * (double)th;
-            double pz = ray.oz + dz * inv * (double)th;
+            p = f3{__builtin_fmaf(ray.d.x, th, p.x), __builtin_fmaf(ray.d.y, th, p.y),
+                   __builtin_fmaf(ray.d.z, th, p.z)};
+            w = ray_w(ray.bf, ray.wo, th);
             float scat = co.sr + co.sas;
             float ext = scat + co.so + co.saa;
             if (ext <= 0.0f || scat <= 0.0f) break;
@@ -145,8 +148,7 @@ __device__ float trace_path(const DevScene &s, uint32_t x, uint32_t y, uint32_t 
                 float a1 = rng_next(rng);
                 f3 ld = sun_cone_dir(s, a0, a1);
                 Ray sr;
-                ray_setup(s, global_shells(s), px + (double)ld.x * kRayEpsilonKm, py + (double)ld.y * kRayEpsilonKm,
-                          pz + (double)ld.z * kRayEpsilonKm, ld, sr);
+                ray_setup(s, global_shells(s), p, w, ld, kRayEpsilonKm, sr);
                 if (sr.hit && sr.kind == 1) {
                     float T = transmittance(s, band, l, maj_row, sr, rng, wd);
                     float ph = mixed_phase(s, co.sr, co.sas, dot3(ray.d, ld));
@@ -173,20 +175,15 @@ __device__ float trace_path(const DevScene &s, uint32_t x, uint32_t y, uint32_t 
             }
             tp = tp * w;
             if (!roulette(s, tp, depth, rng)) break;
-            ox = px + (double)nd.x * kRayEpsilonKm;
-            oy = py + (double)nd.y * kRayEpsilonKm;
-            oz = pz + (double)nd.z * kRayEpsilonKm;
             dir = nd;
         } else if (ray.kind == 1) {
             if (depth == 0 && in_sun_cone(s, dir)) L = L + tp * l.lsun;
             break;
         } else {
-            double dx = ray.d.x, dy = ray.d.y, dz = ray.d.z;
-            double inv = inv_norm64(dx, dy, dz);
-            double px = ray.ox + dx * inv * (double)ray.tmax;
-            double py = ray.oy + dy * inv * (double)ray.tmax;
-            double pz = ray.oz + dz * inv * (double)ray.tmax;
-            f3 nrm = normalize_or_y(f3{(float)px, (float)py, (float)pz});
+            p = f3{__builtin_fmaf(ray.d.x, ray.tmax, p.x), __builtin_fmaf(ray.d.y, ray.tmax, p.y),
+                   __builtin_fmaf(ray.d.z, ray.tmax, p.z)};
+            w = 0.0f;
+            f3 nrm = normalize_or_y(p);
             // ground_radiance (wgsl:587-608)
             float sum = 0.0f;
             for (uint32_t i = 0; i < n_light; ++i) {
@@ -196,8 +193,7 @@ __device__ float trace_path(const DevScene &s, uint32_t x, uint32_t y, uint32_t 
                 float cs = fmaxf(dot3(nrm, ld), 0.0f);
                 if (cs > 0.0f) {
                     Ray sr;
-                    ray_setup(s, global_shells(s), px + (double)ld.x * kRayEpsilonKm, py + (double)ld.y * kRayEpsilonKm,
-                              pz + (double)ld.z * kRayEpsilonKm, ld, sr);
+                    ray_setup(s, global_shells(s), p, w, ld, kRayEpsilonKm, sr);
                     if (sr.hit && sr.kind == 1) {
                         float T = transmittance(s, band, l, maj_row, sr, rng, wd);
                         sum = sum + s.albedo * kInvPi * l.lsun * T * cs / s.cone_pdf;
@@ -216,9 +212,6 @@ __device__ float trace_path(const DevScene &s, uint32_t x, uint32_t y, uint32_t 
             if (pdf <= 0.0f || co_out <= 0.0f) break;
             tp = tp * s.albedo * kInvPi * co_out / pdf;
             if (!roulette(s, tp, depth, rng)) break;
-            ox = px + (double)nd.x * kRayEpsilonKm;
-            oy = py + (double)nd.y * kRayEpsilonKm;
-            oz = pz + (double)nd.z * kRayEpsilonKm;
             dir = nd;
         }
         if (L < 0.0f || L != L || tp < 0.0f || tp != tp) return 0.0f;

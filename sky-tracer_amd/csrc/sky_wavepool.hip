@@ -61,10 +61,12 @@ __device__ __forceinline__ const DevJob &job_at(const DevJob *jobs, uint32_t j)
     return *(const DevJob *)(cj + j);
 }
 
-// Everything a lane keeps between loop trips.  The main ray starts at the
-// current vertex (px, py, pz): the camera position, or the last vertex moved
-// by the ray epsilon, exactly as the reference offsets new rays
-// (sky_trace.wgsl:744, 766).  During a shadow ray the vertex stays put.
+// Everything a lane keeps between loop trips.  The current vertex is
+// (px, py, pz) with r^2 - R^2 = w carried exactly (sky_device.hpp): the
+// camera, a collision or a ground hit, and after a bounce the origin of the
+// new main ray (the vertex moved by the ray epsilon, as the reference offsets
+// new rays, sky_trace.wgsl:744, 766).  During a shadow ray the vertex stays
+// put.
 struct LaneCtx {
     int st, ev, mode, vtype, kind;
     uint32_t slot, band;
@@ -72,14 +74,13 @@ struct LaneCtx {
     Rng rng;
     float tp, L;
     uint32_t depth;
-    double px, py, pz;  // current vertex / main-ray origin (FP64)
+    float px, py, pz, w;  // current vertex / main-ray origin and its r^2 - R^2
     f3 vin;             // incoming direction at the vertex (camera direction at start)
     float nee_sum, nee_factor, rw;
     uint32_t nee_i;
     // ray being tracked
     f3 d;
-    float bf, qf, tmax;
-    double q;
+    float bf, qf, wo, tmax;  // the ray's b, q and origin w (sky_device.hpp)
     float t, tau, seg_end, maj;
     float rm;  // FAST kernels: 1 / maj, from the LDS majorant table
     float aux;  // shadow-ray transmittance (0/1 delta, product for ratio tracking)
@@ -198,12 +199,12 @@ __device__ __forceinline__ void add_dlog_phase(const DevScene &s, const AdjArgs 
 // removes a global-load latency from every shell crossing.
 constexpr uint32_t kMajLdsMax = 2048;
 // Per-wave LDS layout (floats): sums [256], sums of squares [256], adjoint
-// gradient [16 doubles], each lane's 1 / |d| of its current ray [64 doubles],
-// the ring of path starts [64 x 4], majorant grid [maj_lds] (in the FAST
-// kernels as (majorant, reciprocal) pairs), shell table [shell_lds doubles].  The paths' gradient contributions cancel heavily
+// gradient [16 doubles], the ring of path starts [64 x 4], majorant grid
+// [maj_lds] (in the FAST kernels as (majorant, reciprocal) pairs), shell
+// table [shell_lds].  The paths' gradient contributions cancel heavily
 // across paths (scores of both signs), so the item's sum is kept in FP64.
 constexpr uint32_t kRingSlots = 64;
-constexpr uint32_t kLdsFixed = 2 * kSlotsPerWave + 32 + 128 + 4 * kRingSlots;
+constexpr uint32_t kLdsFixed = 2 * kSlotsPerWave + 32 + 4 * kRingSlots;
 
 typedef __attribute__((address_space(3))) const float *LdsFloatPtr;
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -237,9 +238,6 @@ struct MajSrc {
         const v4f u = units[band];
         return make_float4(u.x, u.y, u.z, u.w);
     }
-    // FAST kernels: 1 / |d| (FP64) of each lane's current ray, kept from its
-    // set-up for the vertex update at the ray's end
-    __attribute__((address_space(3))) double *inv;
 };
 
 __device__ __forceinline__ float majorant(const MajSrc &m, uint32_t L, uint32_t band, int k)
@@ -258,17 +256,15 @@ __device__ __forceinline__ void seg_begin(const DevScene &s, const MajSrc &m, La
 {
     float tc;
     if constexpr (MJ == 2) {
-        // the shell's majorant and its reciprocal first: the LDS read overlaps
-        // the crossing's FP64 arithmetic
         const v2f mm = m.lds2[c.band * s.L + (uint32_t)c.k];
         c.maj = mm.x;
         c.rm = mm.y;
-        tc = next_crossing_sel(s, m.shells.lds, c.q, c.bf, c.k, c.out, c.kn, c.on);
+        tc = next_crossing_sel(s, m.shells.lds, c.qf, c.bf, c.k, c.out, c.kn, c.on);
     } else if constexpr (MJ == 1) {
         c.maj = m.lds[c.band * s.L + (uint32_t)c.k];
-        tc = next_crossing_sel(s, m.shells.lds, c.q, c.bf, c.k, c.out, c.kn, c.on);
+        tc = next_crossing_sel(s, m.shells.lds, c.qf, c.bf, c.k, c.out, c.kn, c.on);
     } else {
-        tc = next_crossing(s, m.shells, c.q, c.bf, c.k, c.out, c.kn, c.on);
+        tc = next_crossing(s, m.shells, c.qf, c.bf, c.k, c.out, c.kn, c.on);
         c.maj = majorant(m, s.L, c.band, c.k);
     }
     // min(max(tc, t), tmax) as one median: every caller has t < tmax (a ray
@@ -277,19 +273,17 @@ __device__ __forceinline__ void seg_begin(const DevScene &s, const MajSrc &m, La
     c.seg_end = __builtin_amdgcn_fmed3f(tc, c.t, c.tmax);
 }
 
-// Sets up a ray from (ox, oy, oz) along d (FP64 set-up, sky_device.hpp) and
+// Sets up a ray from the vertex moved by e along d (sky_device.hpp) and
 // starts tracking it; returns false if it has no boundary.
 template <int MJ>
-__device__ __forceinline__ bool ray_begin(const DevScene &s, const MajSrc &m, LaneCtx &c, double ox,
-                                          double oy, double oz, f3 d, int mode)
+__device__ __forceinline__ bool ray_begin(const DevScene &s, const MajSrc &m, LaneCtx &c, f3 d, float e, int mode)
 {
     Ray r;
-    ray_setup<MJ ? (int)kFastMaxShells : 0>(s, m.shells, ox, oy, oz, d, r);
-    if constexpr (MJ != 0) m.inv[lane_id()] = r.inv;
+    ray_setup<MJ ? (int)kFastMaxShells : 0>(s, m.shells, f3{c.px, c.py, c.pz}, c.w, d, e, r);
     c.d = d;
     c.bf = r.bf;
     c.qf = r.qf;
-    c.q = r.q;
+    c.wo = r.wo;
     c.tmax = r.tmax;
     c.kind = r.kind;
     c.mode = mode;
@@ -307,7 +301,7 @@ __device__ __forceinline__ bool ray_begin(const DevScene &s, const MajSrc &m, La
 
 __device__ __forceinline__ float lane_altitude(const DevScene &s, const LaneCtx &c, float t)
 {
-    return altitude_bq(s, c.bf, c.qf, t);
+    return altitude_bw(s, c.bf, c.wo, t);
 }
 
 template <bool FAST>
@@ -525,14 +519,14 @@ __device__ __forceinline__ void track_step_fast(const DevScene &s, const MajSrc 
     }
 }
 
-// Vertex at distance t along the main ray from (px, py, pz), FP64.
-__device__ __forceinline__ void advance_vertex(LaneCtx &c, float t)
+// Vertex at distance t along the main ray from (px, py, pz): r^2 - R^2 is
+// the ray's w at t, or exactly 0 on the ground.
+__device__ __forceinline__ void advance_vertex(LaneCtx &c, float t, bool ground)
 {
-    double dx = c.d.x, dy = c.d.y, dz = c.d.z;
-    double f = inv_norm64(dx, dy, dz) * (double)t;
-    c.px += dx * f;
-    c.py += dy * f;
-    c.pz += dz * f;
+    c.px = __builtin_fmaf(c.d.x, t, c.px);
+    c.py = __builtin_fmaf(c.d.y, t, c.py);
+    c.pz = __builtin_fmaf(c.d.z, t, c.pz);
+    c.w = ground ? 0.0f : ray_w(c.bf, c.wo, t);
 }
 
 enum Next : int { NX_NEE = 1, NX_CONT = 2, NX_MAIN = 3, NX_FINISH = 4 };
@@ -574,7 +568,7 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
 
     // 1. the event; a ground hit or an accepted collision moves the vertex
     const bool at_ground = c.ev == EV_BOUNDARY && c.kind == 0;
-    if (at_ground || c.ev == EV_HIT) advance_vertex(c, at_ground ? c.tmax : c.t);
+    if (at_ground || c.ev == EV_HIT) advance_vertex(c, at_ground ? c.tmax : c.t, at_ground);
     if (c.ev == EV_BOUNDARY) {
         if (c.kind == 1) {
             // exit: the sun disk is seen only by the camera ray (wgsl:745-749)
@@ -636,7 +630,7 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
     //    azimuth negated), cosine lobe (sample_cosine_hemisphere, wgsl:548-560)
     f3 dir = c.vin;
     f3 nrm{0.0f, 1.0f, 0.0f};
-    if (c.vtype == 1) nrm = normalize_or_y(f3{(float)c.px, (float)c.py, (float)c.pz});
+    if (c.vtype == 1) nrm = normalize_or_y(f3{c.px, c.py, c.pz});
     if (nx == NX_NEE || nx == NX_CONT) {
         f3 axis;
         float mu, st, phi;
@@ -733,10 +727,9 @@ __device__ __forceinline__ void handle_event(const DevScene &s, const MajSrc &m,
                         a.J = 0.0f;
                     }
                 } else {
+                    // the ray stage starts the new main ray the ray epsilon
+                    // from the vertex (depth > 0)
                     c.depth++;
-                    c.px += (double)dir.x * kRayEpsilonKm;
-                    c.py += (double)dir.y * kRayEpsilonKm;
-                    c.pz += (double)dir.z * kRayEpsilonKm;
                     nx = NX_MAIN;
                 }
             }
@@ -795,13 +788,11 @@ __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSr
     {
         // advance_vertex for a ground hit or an accepted collision (a lane
         // that does not move adds d * 0, which leaves its position exact)
-        // (1 / |d| as the ray's set-up computed it)
-        const double dx = c.d.x, dy = c.d.y, dz = c.d.z;
-        const double f = m.inv[lane_id()] * (double)(at_ground ? c.tmax : c.t);
-        const double fm = at_ground || hit ? f : 0.0;
-        c.px += dx * fm;
-        c.py += dy * fm;
-        c.pz += dz * fm;
+        const float tm = at_ground ? c.tmax : (hit ? c.t : 0.0f);
+        c.px = __builtin_fmaf(c.d.x, tm, c.px);
+        c.py = __builtin_fmaf(c.d.y, tm, c.py);
+        c.pz = __builtin_fmaf(c.d.z, tm, c.pz);
+        c.w = at_ground ? 0.0f : (hit ? ray_w(c.bf, c.wo, c.t) : c.w);
     }
     // exit: the sun disk is seen only by the camera ray (wgsl:745-749); the
     // ray directions are unit vectors already (camera_dir, frame_dir)
@@ -849,7 +840,7 @@ __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSr
     // 3. one direction: sun cone (NEE), Rayleigh/HG lobe (IS) or isotropic
     //    (sample_isotropic, wgsl:534-539, as a frame around +y with the
     //    azimuth negated) at a scattering vertex, cosine lobe at the ground
-    const f3 nrm = normalize_or_y(f3{(float)c.px, (float)c.py, (float)c.pz});
+    const f3 nrm = normalize_or_y(f3{c.px, c.py, c.pz});
     // sun cone (sun_cone_mu): mu = 1 - x, sin = sqrt(x (2 - x))
     const float x_sun = fminf(u0, 0.99999994f) * s.omc;
     float mu_v0, phi_v;
@@ -913,13 +904,9 @@ __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSr
         c.L = live && bad ? 0.0f : Lc;
         c.tp = ok ? (killed ? tp1 : tp2) : c.tp;
     }
+    // (the ray stage starts the bounce's main ray the ray epsilon from the
+    // vertex: depth > 0)
     c.depth = bounce ? c.depth + 1u : c.depth;
-    {
-        const double e = bounce ? (double)kRayEpsilonKm : 0.0;
-        c.px += (double)dir.x * e;
-        c.py += (double)dir.y * e;
-        c.pz += (double)dir.z * e;
-    }
     nx = is_cont ? (bounce ? NX_MAIN : NX_FINISH) : nx;
     // loop top of trace_path (wgsl:709-717) plus the depth cap
     bool main_ray = false;
@@ -947,18 +934,25 @@ __device__ __forceinline__ void handle_event_fast(const DevScene &s, const MajSr
 }
 
 // Second half of a handler pass: one ray set-up for every lane in RAY -- a
-// shadow ray from the vertex moved by the ray epsilon (wgsl:744, 766) or a
-// main ray from the current origin (vertex already moved, or the camera).
-// A shadow ray that does not leave the atmosphere is a zero sun sample; a
-// main ray with no boundary ends the path at the next pass.
+// shadow ray, or a main ray after a bounce, from the vertex moved by the ray
+// epsilon (wgsl:744, 766), or the camera ray from the camera (depth 0).  A
+// main ray's origin becomes the path's vertex.  A shadow ray that does not
+// leave the atmosphere is a zero sun sample; a main ray with no boundary
+// ends the path at the next pass.
 template <int MJ>
 __device__ __forceinline__ void ray_pass(const DevScene &s, const MajSrc &m, LaneCtx &c)
 {
     const bool shadow = c.mode != kTrackCollision;
     const f3 dir = c.d;
-    const double e = shadow ? (double)kRayEpsilonKm : 0.0;
-    bool hit = ray_begin<MJ>(s, m, c, c.px + (double)dir.x * e, c.py + (double)dir.y * e, c.pz + (double)dir.z * e, dir,
-                         c.mode);
+    const float e = shadow || c.depth > 0u ? kRayEpsilonKm : 0.0f;
+    bool hit = ray_begin<MJ>(s, m, c, dir, e, c.mode);
+    {
+        const float em = shadow ? 0.0f : e;
+        c.px = __builtin_fmaf(dir.x, em, c.px);
+        c.py = __builtin_fmaf(dir.y, em, c.py);
+        c.pz = __builtin_fmaf(dir.z, em, c.pz);
+        c.w = shadow ? c.w : c.wo;
+    }
     if (shadow) {
         if (!(hit && c.kind == 1)) {
             c.st = ST_EVENT;
@@ -1031,9 +1025,10 @@ __device__ __forceinline__ void start_path(const DevScene &s, LaneCtx &c, AdjLan
     c.rng = rng_make(pixel, it.s0 + si, c.band, s);
     c.rng.w += 2u * kLcgMul;
     c.vin = f3{e.x, e.y, e.z};
-    c.px = 0.0;
-    c.py = s.obs_y;
-    c.pz = 0.0;
+    c.px = 0.0f;
+    c.py = s.obs_yf;
+    c.pz = 0.0f;
+    c.w = s.obs_wf;
     c.tp = 1.0f;
     c.L = 0.0f;
     c.depth = 0;
@@ -1170,16 +1165,15 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
     // wave index, read into a scalar register so that the wave's LDS bases are
     // scalars (the replay kernels, short of scalar registers, keep it per lane)
     const uint32_t wid = ADJ ? threadIdx.x >> 6 : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    float *wave_lds = dyn_lds + wid * (kLdsFixed + a.maj_lds + 2 * a.shell_lds + a.lam_lds);
+    float *wave_lds = dyn_lds + wid * (kLdsFixed + a.maj_lds + a.shell_lds + a.lam_lds);
     float *acc_sum = wave_lds;
     float *acc_sq = wave_lds + kSlotsPerWave;
     double *acc_grad = (double *)(wave_lds + 2 * kSlotsPerWave);
-    double *lds_inv = (double *)(wave_lds + 2 * kSlotsPerWave + 32);
-    const LdsRingPtr ring = (LdsRingPtr)(wave_lds + 2 * kSlotsPerWave + 32 + 128);
+    const LdsRingPtr ring = (LdsRingPtr)(wave_lds + 2 * kSlotsPerWave + 32);
     float *lds_maj = wave_lds + kLdsFixed;
-    double *lds_shell = (double *)(lds_maj + a.maj_lds);
+    float *lds_shell = lds_maj + a.maj_lds;
     // FAST: [band] 4-float coefficients, then [band] sun radiance (kMaxLambda bands each)
-    v4f *lds_lam4 = (v4f *)(lds_maj + a.maj_lds + 2 * a.shell_lds);
+    v4f *lds_lam4 = (v4f *)(lds_maj + a.maj_lds + a.shell_lds);
     float *lds_lsun = (float *)(lds_lam4 + (uint32_t)kMaxLambda);
     // FAST replays: [band] A.lam_units after the sun radiance
     v4f *lds_units = (v4f *)(lds_lsun + (uint32_t)kMaxLambda);
@@ -1229,13 +1223,12 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
         msrc.glob = (GlobalFloatPtr)s.maj;
         const uint32_t nmaj = s.nlam * s.L;
         msrc.use_lds = FAST || (nmaj <= a.maj_lds && s.L + 1 <= a.shell_lds);
-        msrc.shells.lds = (LdsDoublePtr)lds_shell;
-        msrc.shells.glob = (GlobalDoublePtr)s.shell_d;
+        msrc.shells.lds = (LdsShellPtr)lds_shell;
+        msrc.shells.glob = (GlobalShellPtr)s.shell_w;
         msrc.shells.use_lds = msrc.use_lds;
         msrc.lam4 = (LdsFloat4Ptr)lds_lam4;
         msrc.lsun = (LdsFloatPtr)lds_lsun;
         msrc.units = (LdsFloat4Ptr)lds_units;
-        msrc.inv = (__attribute__((address_space(3))) double *)lds_inv;
         if (msrc.use_lds) {
             if (loaded != jid) {
                 if constexpr (MJ == 2) {
@@ -1246,7 +1239,7 @@ __device__ __forceinline__ void wavepool_body(const PoolArgs &a, const AdjArgs &
                 } else {
                     for (uint32_t i = lane; i < nmaj; i += 64) lds_maj[i] = s.maj[i];
                 }
-                for (uint32_t i = lane; i <= s.L; i += 64) lds_shell[i] = s.shell_d[i];
+                for (uint32_t i = lane; i <= s.L; i += 64) lds_shell[i] = s.shell_w[i];
                 if constexpr (FAST) {
                     for (uint32_t i = lane; i < s.nlam; i += 64) {
                         const DevLambda l = s.lam[i];
@@ -1557,7 +1550,7 @@ WavePlan plan_launch(const sky_tuning &t, int num_cus, uint32_t total_items, uin
     p.shell_lds = p.maj_lds ? ((max_layers + 1u + 7u) & ~7u) : 0u;
     if (p.fast == 1 || p.fast == 2) p.maj_lds *= 2;  // (majorant, reciprocal) pairs
     p.lam_lds = p.fast ? 5u * (uint32_t)kMaxLambda : 0u;
-    p.lds_bytes = kWavesPerBlock * (kLdsFixed + p.maj_lds + 2 * p.shell_lds + p.lam_lds) * (uint32_t)sizeof(float);
+    p.lds_bytes = kWavesPerBlock * (kLdsFixed + p.maj_lds + p.shell_lds + p.lam_lds) * (uint32_t)sizeof(float);
     uint32_t wpc = t.waves_per_cu ? t.waves_per_cu : kDefaultWavesPerCu;
     uint64_t waves = (uint64_t)(num_cus > 0 ? num_cus : 256) * wpc;
     uint64_t w = waves < total_items ? waves : total_items;

@@ -7,6 +7,8 @@ alike up to float rounding):
   (config.rs:47 defaults to 0.2 km);
 * the sun at the zenith, below the horizon and just above it at oblique
   azimuths;
+* the sky-view LUT row on the horizon, whose camera rays graze the ground
+  (the ray set-up's FP64 branch);
 * one-pixel and odd-sized images with 1, 2, 4, 5 and 64 bands, whose XYZ takes
   the wave-level band sums (power-of-two band counts from 4) or the band-order
   sum on one lane (the others), both for renders that hold every sample in one
@@ -42,6 +44,28 @@ def _pair(gpu_context, scene, cfg, op):
 def test_observer_altitudes_pair_with_oracle(gpu_context, h):
     scene, cfg, op = case(width=64, height=32, n_lambda=8, spp=1, observer_altitude_km=h, **FAST)
     _, frac, _ = _pair(gpu_context, scene, cfg, op)
+    assert frac >= 0.995
+
+
+@pytest.mark.parametrize("h", [1.0, 20.0])
+def test_sky_view_horizon_row_grazes_the_ground(gpu_context, h):
+    """A 33-row sky-view LUT has its v = 0.75 row (y = 24) on the horizon
+    (sky_trace.wgsl:625-681): camera rays tangent to the ground, the rays the
+    ray set-up forms in FP64 (the oracle counts exactly the 256 of that row).
+    The other rows pair path for path.  On the tangent row the GPU's and the
+    oracle's FP32 camera directions (hardware against library cosines, ~1 ulp
+    apart) can fall on either side of tangency, so that row is checked finite
+    and non-negative only."""
+    scene, cfg, op = case(width=32, height=33, n_lambda=8, spp=1, projection=1, observer_altitude_km=h, **FAST)
+    print(f"[gpu] sky-view 32x33 nl=8 h_obs={h}", flush=True)
+    g = pysky.render(scene, cfg, context=gpu_context)
+    ref, _, _, cnt, rc = O.render(op, "f32")
+    assert rc == 0 and cnt["graze64"] == 32 * 8
+    film = g.values().reshape(33, 32, 8)
+    assert np.all(np.isfinite(film)) and np.all(film >= 0)
+    rows = np.arange(33) != 24
+    frac, bad = paired_stats(film[rows], ref[rows])
+    print(f"paired fraction off the horizon row {frac:.5f} ({len(bad)} differ)", flush=True)
     assert frac >= 0.995
 
 

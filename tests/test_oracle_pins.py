@@ -149,6 +149,106 @@ def test_grazing_boundary_closed_form():
         assert t == pytest.approx(math.sqrt(6460.0 ** 2 - (6360.0 + h) ** 2), rel=1e-12)
 
 
+def _exact_setup(p, o, d, e):
+    """next_boundary and the walk's first shell in double for the FP32
+    tracer's view of a vertex: the position rounded to FP32, r^2 - R^2 of the
+    double position rounded to FP32, the FP32 direction renormalised."""
+    R = p.ground_radius_km
+    Rt = p.atmosphere_radius_km
+    p32 = np.asarray(o, np.float32).astype(np.float64)
+    w = float(np.float32(np.dot(o, o) - R * R))
+    d32 = np.asarray(d, np.float32).astype(np.float64)
+    d64 = d32 / np.linalg.norm(d32)
+    b0 = float(np.dot(p32, d64))
+    q = w - b0 * b0
+    b = b0 + e
+    wo = w + e * (2.0 * b0 + e)
+    eps = 0.005
+    tg = -1.0
+    if q <= 0.0:
+        t0, t1 = -b - math.sqrt(-q), -b + math.sqrt(-q)
+        tg = t0 if t0 > eps else (t1 if t1 > eps else -1.0)
+    t_exit = max(-b + math.sqrt((Rt - R) * (Rt + R) - q), eps)
+    kind, t = (0, tg) if (tg > eps and tg < t_exit) else (1, t_exit)
+    H = O.shell_heights(p)
+    D = H * (2.0 * R + H)
+    layer = int(np.sum(D[1:-1] <= wo))
+    return dict(kind=kind, t_max=t, layer=layer, b=b, q=q, wo=wo)
+
+
+def test_fp32_ray_setup_matches_double_geometry():
+    """The FP32 tracer's ray set-up (b from the FP32 vertex, q = w - b^2 from
+    the carried r^2 - R^2, the stable near ground root) against the same
+    geometry in double: b within its stated rounding bound 3 2^-24 |p|, q
+    within the grazing tolerance, the same kind and shell, roots within a
+    metre, and the near ground root from a vertex metres above the ground to
+    1e-5 relative (the unstable -b - sqrt(-q) is off by several % there)."""
+    p = O.Params()
+    R = p.ground_radius_km
+    rng = np.random.default_rng(5)
+    n_checked = 0
+    for h in (0.0, 0.005, 0.2, 1.0, 5.3, 25.7, 80.1, 99.9):
+        for i in range(60):
+            n = rng.normal(size=3)
+            o = n / np.linalg.norm(n) * (R + h)
+            d = rng.normal(size=3)
+            if i % 4 == 0:  # steeply down
+                d = -o / np.linalg.norm(o) + 0.05 * d
+            d = d / np.linalg.norm(d)
+            for e in (0.0, 0.005):
+                got, graze = O.ray_setup_f32(p, o, d, e)
+                ref = _exact_setup(p, o, d, e)
+                bound = 3 * 2.0 ** -24 * np.linalg.norm(o)
+                assert abs(got["b"] - ref["b"]) <= bound + 1e-9, (h, i, e)
+                ab = abs(ref["b"] - e)
+                assert abs(got["q"] - ref["q"]) <= ab * (2.4e-7 * ab + 2.4e-3) + 1e-5, (h, i, e)
+                assert got["kind"] == ref["kind"] and got["layer"] == ref["layer"], (h, i, e, got, ref)
+                assert abs(got["t_max"] - ref["t_max"]) <= 1e-6 * ref["t_max"] + 1e-3, (h, i, e, got, ref)
+                assert not graze
+                n_checked += 1
+    assert n_checked == 8 * 60 * 2
+    # near ground root from vertices 7 and 20 m up (beyond the 5 m ray
+    # epsilon): t0 = w_o / (sqrt(-q) - b)
+    for h in (0.007, 0.02):
+        for mu in (-1.0, -0.7, -0.3, -0.1):
+            o = np.array([0.0, R + h, 0.0])
+            d = np.array([math.sqrt(1.0 - mu * mu), mu, 0.0])
+            got, _ = O.ray_setup_f32(p, o, d)
+            ref = _exact_setup(p, o, d, 0.0)
+            assert got["kind"] == 0 and abs(got["t_max"] / ref["t_max"] - 1.0) < 1e-5, (h, mu, got, ref)
+
+
+def test_fp32_ray_setup_is_exact_at_ground_tangency():
+    """Rays within centimetres of tangency to the ground take the FP64 branch,
+    and hit or miss exactly as the double geometry says; the FP32 q alone
+    (error ~2|b| 1.2 m) would decide about half of them wrongly."""
+    p = O.Params()
+    R = p.ground_radius_km
+    rng = np.random.default_rng(9)
+    n_graze = 0
+    for h in (0.3, 2.0, 10.0, 40.0):
+        for k in range(12):
+            phi = rng.uniform(0, 2 * math.pi)
+            up = np.array([0.0, 1.0, 0.0])
+            o = (R + h) * np.array([math.sin(phi) * 0.6, math.cos(phi), math.sin(phi) * 0.8])
+            u = o / np.linalg.norm(o)
+            t = np.cross(u, up if abs(u[1]) < 0.9 else np.array([1.0, 0.0, 0.0]))
+            t /= np.linalg.norm(t)
+            # dip below the horizontal by the tangent angle, nudged by a few
+            # centimetres of miss distance either way
+            dip = math.acos(R / (R + h)) + rng.uniform(-1.0, 1.0) * 2e-5 / (R + h)
+            d = math.cos(dip) * t - math.sin(dip) * u
+            d32 = np.asarray(d, np.float32).astype(np.float64)
+            ref = _exact_setup(p, o, d32, 0.0)
+            if abs(ref["q"]) < 2e-3:
+                continue  # too close to call in double from the rounded inputs
+            got, graze = O.ray_setup_f32(p, o, d32)
+            assert graze, (h, k, ref["q"])
+            assert got["kind"] == ref["kind"], (h, k, ref["q"], got, ref)
+            n_graze += 1
+    assert n_graze >= 30
+
+
 # ---------------------------------------------------------- medium, majorants
 
 def test_medium_closed_form():
