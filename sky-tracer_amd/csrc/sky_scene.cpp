@@ -602,6 +602,11 @@ sky_status build_scene(const sky_planet *planet, const sky_atmosphere *a, const 
     s.watchdog_limit = cfg->watchdog_limit;
     s.max_segments = cfg->max_segments;
     s.flags = cfg->flags;
+    // shell crossings per tracking step of the general kernels: L / 4 within
+    // [3, 8] (on 1024x512x64 renders: 9 shells 310 -> 255 ms per 64-spp
+    // launch, 16 shells 235 -> 157 ms and 32 shells 529 -> 287 ms per 32-spp
+    // launch, 64 shells 338 -> 163 ms per 16-spp launch; the paths are the same)
+    s.walk_cap = std::min(std::max(s.L / 4u, 3u), 8u);
     return SKY_OK;
 }
 

@@ -326,6 +326,7 @@ __device__ __forceinline__ void track_step(const DevScene &s, const MajSrc &m, L
 {
     float tn = 0.0f;
     const bool carried = (s.flags & kFlagCarriedTau) != 0;
+    uint32_t walked = 0;
     for (;;) {
         if (c.t < c.seg_end) {
             if (c.tau < 0.0f) c.tau = free_flight(rng_next(c.rng));
@@ -356,6 +357,10 @@ __device__ __forceinline__ void track_step(const DevScene &s, const MajSrc &m, L
         c.k = c.kn;
         c.out = c.on;
         seg_begin<0>(s, m, c);
+        // at most walk_cap crossings per step: the lane goes on from here in
+        // the next step (same draws, same order), so the wave is not held by
+        // its longest walk
+        if (++walked == s.walk_cap) return;
     }
     c.t = tn;
     const float h = lane_altitude(s, c, tn);
