@@ -69,7 +69,7 @@ struct DevScene {
     uint32_t seed_lo, seed_hi;
     uint32_t nlam, L;
     uint32_t dls, projection, max_depth, rr_start, watchdog_limit, max_segments, flags;
-    uint32_t walk_cap;  // shell crossings per tracking step of the general kernels (0: no cap)
+    uint32_t walk_cap;  // shell crossings a lane takes per tracking step (0: no cap)
     // Tabulated medium (null tab: the analytic profiles): sigma (R, As, O,
     // Aa) per band at K altitude breakpoints, linear in between; a uniform
     // index of tab_nidx cells gives each altitude's first candidate interval.

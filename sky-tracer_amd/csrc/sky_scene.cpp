@@ -602,11 +602,14 @@ sky_status build_scene(const sky_planet *planet, const sky_atmosphere *a, const 
     s.watchdog_limit = cfg->watchdog_limit;
     s.max_segments = cfg->max_segments;
     s.flags = cfg->flags;
-    // shell crossings per tracking step of the general kernels: L / 4 within
-    // [3, 8] (on 1024x512x64 renders: 9 shells 310 -> 255 ms per 64-spp
-    // launch, 16 shells 235 -> 157 ms and 32 shells 529 -> 287 ms per 32-spp
-    // launch, 64 shells 338 -> 163 ms per 16-spp launch; the paths are the same)
-    s.walk_cap = std::min(std::max(s.L / 4u, 3u), 8u);
+    // shell crossings a lane takes per tracking step (the general kernels and
+    // the ratio-tracked specialised ones): 2 up to 4 shells, 4 up to 16, then
+    // 8 (on 1024x512x64 renders, the paths unchanged: ratio tracking in 4
+    // shells growing 2.5x 385 -> 370 ms per 128-spp launch; 8 uniform shells
+    // 310 -> 249 ms and 9 shells 310 -> 277 ms per 64-spp launch, 16 shells
+    // 235 -> 157 ms and 32 shells 529 -> 287 ms per 32-spp launch, 64 shells
+    // 338 -> 163 ms per 16-spp launch)
+    s.walk_cap = s.L <= 4u ? 2u : (s.L <= 16u ? 4u : 8u);
     return SKY_OK;
 }
 

@@ -424,7 +424,13 @@ __device__ __forceinline__ void track_step_fast(const DevScene &s, const MajSrc 
 {
     float tn = 0.0f;
     if (c.tau < 0.0f && c.t < c.seg_end) c.tau = free_flight(rng_next(c.rng));
-    bool ended = false;
+    // the walk stops at a boundary: the ray's end, the watchdog (no crossings
+    // left) or, in the ratio-tracked kernels, this step's share of crossings
+    // (walk_cap: the lane resumes at the boundary in the next step, with the
+    // same draws in the same order; compiled only there, where it pays)
+    bool stop = false;
+    uint32_t floor = 0u;
+    if constexpr (RATIO) floor = s.walk_cap && c.guard > s.walk_cap ? c.guard - s.walk_cap : 0u;
     // 1 / majorant of the current segment (the table's f_rcp(maj)); the loop
     // leaves with the segment of the collision or of the ray's end, whose
     // ratio below reuses it
@@ -445,8 +451,8 @@ __device__ __forceinline__ void track_step_fast(const DevScene &s, const MajSrc 
             c.tau = nonempty ? -1.0f : c.tau;
         }
         c.t = c.seg_end;
-        if ((c.t >= c.tmax) | (c.guard == 0u)) {
-            ended = true;
+        if ((c.t >= c.tmax) | (c.guard <= floor)) {
+            stop = true;
             break;
         }
         c.guard--;
@@ -460,7 +466,9 @@ __device__ __forceinline__ void track_step_fast(const DevScene &s, const MajSrc 
     // select form: the collision's draw is taken only by lanes that have one.
     // A ratio-tracked shadow ray instead multiplies its transmittance by
     // 1 - sigma_t / majorant and goes on (sky_trace.wgsl:441-475).
-    const bool col = !ended;
+    const bool paused = RATIO && stop && c.t < c.tmax && c.guard != 0u;
+    const bool ended = stop && !paused;
+    const bool col = !stop;
     const bool main = c.mode == kTrackCollision;
     const bool rt = RATIO && c.mode == kTrackRatio;
     // (an ended lane evaluates the medium at its end point, unused)
