@@ -433,7 +433,7 @@ def bench_scan(args):
         "metric": "combo x sample evaluations/s (wavelength-optimizer screening) on 1 MI355X",
         "value": value, "unit": "evaluations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32 (f64 sums)", "data": "synthetic (sky-view LUT bake of this build)",
+        "vs_baseline": None, "dtype": "f32 (f64 sums)", "data": "synthetic (a sky-view LUT bake rendered by libsky_hip)",
         "config": {"workload": f"scan: all {len(combos)} {P}-band combos of the 41-band grid, Voronoi weights, "
                                f"65536 of 655360 sky-view LUT samples ({args.spp or 64}-spp bake), one launch",
                    "parallelism": "1 GPU"},

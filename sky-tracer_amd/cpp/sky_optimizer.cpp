@@ -9,7 +9,7 @@
 //   percentiles, rank by RMSE + 0.25 P95                     main.rs:200-241
 //   ranking.csv, ranking_by_elevation.csv, report.md         main.rs:1203-1402
 // Same flags and outputs as the reference CLI.  Assets are read from the
-// uncompressed EXRs this build's sky_render writes.
+// uncompressed EXRs that sky_render writes.
 #include <algorithm>
 #include <cerrno>
 #include <chrono>
